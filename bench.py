@@ -1,0 +1,169 @@
+"""Benchmark of the GP time-stepping path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (the configuration the headline metric is quoted on): random-phase
+field on a 512^3 periodic grid, dx = 0.5, 4th-order FD, Cash-Karp RKF45 with
+error tolerance 1e-6, V = 0, g = mu = 1.  A step is one accepted rk45 step
+(one solver() call, all attempts included) of the whole field.  The
+compensated FP32 form (FP32 storage and stencil, FP64 reductions, step
+control and psi update) is the headline `value`; the plain FP64 path runs
+on the same input beside it.
+
+`value` is site-updates per second over the K timed steps, field resident
+in HBM.  `roofline` is for the dominant kernel, timed with HIP events on the
+stream it is launched on; its algorithmic bytes are the stage's share of the
+fused-sweep traffic model (RKF45: 27 field passes per attempt, + the low
+word read and write of the compensated form).  `cpu_baseline` is the C
+oracle (a port of the reference's FP64 path) timed on this host on one
+accepted step of the same 512^3 field.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "site-updates/s + % HBM roofline at 512^3 FP32 vs FP64; batched solves/s, p99"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def workload(precision, n=512):
+    from gpe_amd.config import RunConfig, xr_for_spacing
+    return RunConfig(nx=n, ny=n, nz=n, xr=xr_for_spacing(0.5, n), bcs=1, order=4, scheme="rk45",
+                     eqn=4, g=1.0, mu=1.0, tau=0.05, precision=precision)
+
+
+def run_path(precision, psi, steps, warmup):
+    from gpe_amd.sim import Simulation
+    cfg = workload(precision)
+    sim = Simulation(cfg, psi)
+    log(precision, "warmup", warmup)
+    sim.advance(warmup)
+    s0 = sim.state()
+    sim.prof_enable(False)
+    # timed region: the advance call synchronises the stream before returning
+    t0 = time.perf_counter()
+    sim.advance(steps)
+    t1 = time.perf_counter()
+    s1 = sim.state()
+    elapsed = t1 - t0
+    # second pass with per-kernel HIP events (not part of the timed value)
+    sim.prof_enable(True)
+    sim.advance(max(2, min(steps, 10)))
+    prof = sim.prof_query()
+    s2 = sim.state()
+    sim.close()
+    sites = cfg.nx * cfg.ny * cfg.nz
+    return {
+        "elapsed_s": elapsed,
+        "su_per_s": sites * steps / elapsed,
+        "ms_per_step": 1e3 * elapsed / steps,
+        "t_sim": s1["t"] - s0["t"],
+        "dt_last": s1["dt"].real,
+        "prof": prof,
+        "rejects_last": s2["n_rejects"],
+        "sites": sites,
+    }
+
+
+def roofline_of(res):
+    # dominant kernel = largest total event time
+    kind, (n, ms, by) = max(res["prof"].items(), key=lambda kv: kv[1][1])
+    avg_s = ms / n / 1e3
+    achieved = by / avg_s / 1e9
+    return {"bound": "hbm", "kernel": kind, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_launch_ms": round(avg_s * 1e3, 4), "bytes_per_launch": by}
+
+
+def step_roofline(res, bytes_per_site_attempt):
+    # whole-step achieved bandwidth over all stage kernels
+    tot_ms = sum(v[1] for v in res["prof"].values())
+    attempts = res["prof"].get("ck_6", (0,))[0]
+    if not attempts:
+        return None
+    bytes_ = bytes_per_site_attempt * res["sites"] * attempts
+    return round(bytes_ / (tot_ms / 1e3) / 1e9, 1)
+
+
+def cpu_baseline(psi):
+    from oracle import pyoracle
+    cfg = workload("fp64")
+    nt = min(16, os.cpu_count() or 1)
+    st = pyoracle.make_state(cfg.dt)
+    t0 = time.perf_counter()
+    pyoracle.solver(cfg, st, psi, nt)
+    el = time.perf_counter() - t0
+    sites = cfg.nx * cfg.ny * cfg.nz
+    return {"value": round(sites / el, 1), "unit": "site-updates/s", "cores": nt, "kind": "port",
+            "sample": "one accepted rk45 step (%d attempt(s)) of the same 512^3 field, FP64, %.1f s"
+                      % (1 + st.n_rejects, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--seed", type=int, default=20261202)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from gpe_amd import ic
+    t0 = time.perf_counter()
+    psi = ic.random_phase(workload("fp64"), a.seed)
+    log("IC ready in %.1f s" % (time.perf_counter() - t0))
+
+    r32 = run_path("fp32c", psi, a.steps, a.warmup)
+    log("fp32c %.3e su/s, %.3f ms/step" % (r32["su_per_s"], r32["ms_per_step"]))
+    r64 = run_path("fp64", psi, a.steps, a.warmup)
+    log("fp64  %.3e su/s, %.3f ms/step" % (r64["su_per_s"], r64["ms_per_step"]))
+
+    cpu = None if a.no_cpu else cpu_baseline(psi)
+    if cpu:
+        log("cpu baseline %.3e su/s on %d threads" % (cpu["value"], cpu["cores"]))
+
+    out = {
+        "metric": METRIC,
+        "value": round(r32["su_per_s"], 1),
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(r32["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "random-phase 512^3 periodic, dx=0.5, 4th-order FD, RKF45 tol 1e-6, "
+                               "FP32 storage + compensated psi update (FP64 beside)",
+                   "grid": [512, 512, 512], "scheme": "rk45", "order": 4, "bcs": "periodic",
+                   "seed": a.seed},
+        "roofline": roofline_of(r32),
+        "cpu_baseline": cpu,
+        "fp64": {"value": round(r64["su_per_s"], 1), "ms_per_step": round(r64["ms_per_step"], 4),
+                 "roofline": roofline_of(r64)},
+        "fp32c_over_fp64": round(r32["su_per_s"] / r64["su_per_s"], 3),
+        "step_gbs": {"fp32c": step_roofline(r32, 232.0), "fp64": step_roofline(r64, 432.0)},
+        "kernels": {"fp32c": {k: [v[0], round(v[1], 3)] for k, v in r32["prof"].items()},
+                    "fp64": {k: [v[0], round(v[1], 3)] for k, v in r64["prof"].items()}},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

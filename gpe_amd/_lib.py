@@ -1,0 +1,109 @@
+"""ctypes binding of libgpehip.so (the C ABI declared in include/gpe_hip.h).
+
+There is no fallback: if the library is not built, importing a function
+raises.  Build it with `python -m gpe_amd.build`.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgpehip.so")
+
+GPE_OK = 0
+GPE_EULER, GPE_RK2, GPE_RK4, GPE_RK45 = 1, 2, 4, 45
+GPE_FP64, GPE_FP32, GPE_FP32C = 0, 1, 2
+GPE_HOST_C128, GPE_HOST_C64 = 0, 1
+GPE_PROF_KINDS = 16
+
+SCHEME_CODES = {"euler": GPE_EULER, "rk2": GPE_RK2, "rk4": GPE_RK4, "rk45": GPE_RK45}
+PRECISION_CODES = {"fp64": GPE_FP64, "fp32": GPE_FP32, "fp32c": GPE_FP32C}
+
+# Stage kinds, in the order of gpe::Kind (csrc/stage_args.h).
+KIND_NAMES = ["rhs", "euler", "rk2_1", "rk2_2", "rk4_1", "rk4_2", "rk4_3", "rk4_4",
+              "ck_1", "ck_2", "ck_3", "ck_4", "ck_5", "ck_6", "energy"]
+
+# Exported entry points (must match include/gpe_hip.h).
+EXPORTS = [
+    "gpe_last_error", "gpe_ctx_create", "gpe_ctx_destroy", "gpe_solver_step",
+    "gpe_sim_upload", "gpe_sim_download", "gpe_sim_set_state", "gpe_sim_get_state",
+    "gpe_sim_advance", "gpe_sim_norm_history", "gpe_sim_norm", "gpe_sim_energy",
+    "gpe_sim_mass", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
+]
+
+
+class GpeParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("scheme", ctypes.c_int32), ("eqn", ctypes.c_int32), ("bcs", ctypes.c_int32),
+        ("order", ctypes.c_int32), ("precision", ctypes.c_int32), ("real_time", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("xl", ctypes.c_double), ("yl", ctypes.c_double), ("zl", ctypes.c_double),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("g", ctypes.c_double), ("mu", ctypes.c_double),
+        ("omx", ctypes.c_double), ("omy", ctypes.c_double), ("omz", ctypes.c_double),
+        ("urhs", ctypes.c_double), ("diss_amp", ctypes.c_double),
+        ("eps", ctypes.c_double), ("safety", ctypes.c_double),
+        ("dt_increase", ctypes.c_double), ("dt_decrease", ctypes.c_double),
+    ]
+
+
+class GpeStepState(ctypes.Structure):
+    _fields_ = [
+        ("dt_re", ctypes.c_double), ("dt_im", ctypes.c_double),
+        ("t", ctypes.c_double), ("im_t", ctypes.c_double),
+        ("errmax", ctypes.c_double), ("n_rejects", ctypes.c_int32), ("p", ctypes.c_int32),
+    ]
+
+
+class GpeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("gpe error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libgpehip.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libgpehip.so is not built (run `python -m gpe_amd.build`); "
+                           "there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)
+    P, S = ctypes.POINTER(GpeParams), ctypes.POINTER(GpeStepState)
+    sig = {
+        "gpe_last_error": ([], ctypes.c_char_p),
+        "gpe_ctx_create": ([P, ctypes.POINTER(vp)], ctypes.c_int),
+        "gpe_ctx_destroy": ([vp], None),
+        "gpe_solver_step": ([vp, vp, i32, vp, S], ctypes.c_int),
+        "gpe_sim_upload": ([vp, vp, i32], ctypes.c_int),
+        "gpe_sim_download": ([vp, vp, i32], ctypes.c_int),
+        "gpe_sim_set_state": ([vp, S], ctypes.c_int),
+        "gpe_sim_get_state": ([vp, S], ctypes.c_int),
+        "gpe_sim_advance": ([vp, i64, i32], ctypes.c_int),
+        "gpe_sim_norm_history": ([vp, dp, i64], ctypes.c_int),
+        "gpe_sim_norm": ([vp, dp], ctypes.c_int),
+        "gpe_sim_energy": ([vp, dp], ctypes.c_int),
+        "gpe_sim_mass": ([vp, dp], ctypes.c_int),
+        "gpe_sim_renormalise": ([vp, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+        "gpe_sim_rhs": ([vp, vp, i32], ctypes.c_int),
+        "gpe_prof_enable": ([vp, i32], ctypes.c_int),
+        "gpe_prof_query": ([vp, i32, ctypes.POINTER(i64), dp, dp, ctypes.c_char_p], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != GPE_OK:
+        msg = lib().gpe_last_error()
+        raise GpeError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,832 @@
+// gpe_capi.cpp -- C ABI of the GP time-stepping library (include/gpe_hip.h).
+//
+// Host side of the path: owns the device fields, sequences the fused stage
+// kernels for each scheme (solve.f90:57-378), runs the rk45 step-size
+// control of rkqs (solve.f90:158-244) and replaces the reference's
+// solver(var_in, var_out) call (solve.f90:25-53).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/gpe_hip.h"
+#include "launch.h"
+
+using namespace gpe;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) return fail(GPE_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// Division by a real constant in the reference's form (see stage_args.h).
+DivC make_divc(double s) {
+  DivC d{};
+  int e = ilogb(s);
+  d.e = e;
+  d.cs = ldexp(s, -e);
+  d.den = d.cs * d.cs + 0.0 * 0.0;
+  return d;
+}
+
+const char *kind_name(int k) {
+  static const char *names[] = {"rhs",   "euler", "rk2_1", "rk2_2", "rk4_1", "rk4_2", "rk4_3", "rk4_4",
+                                "ck_1",  "ck_2",  "ck_3",  "ck_4",  "ck_5",  "ck_6",  "energy"};
+  return (k >= 0 && k < K_COUNT) ? names[k] : "?";
+}
+
+// Fields read + written per site by each kind in the fused-sweep traffic
+// model (units of one stored complex): RK4 13 passes per step, Cash-Karp 27
+// per attempt (the error scale is counted as register-resident, so the
+// scale field this build keeps is traffic above the model).  The compensated
+// low word (read + write in the final stage) is added by the caller.
+int kind_passes(int k) {
+  switch (k) {
+    case K_RHS: return 2;
+    case K_EULER: return 2;
+    case K_RK2_1: return 2;
+    case K_RK2_2: return 3;
+    case K_RK4_1: return 2;
+    case K_RK4_2: return 4;
+    case K_RK4_3: return 3;
+    case K_RK4_4: return 4;
+    case K_CK_1: return 2;
+    case K_CK_2: return 3;
+    case K_CK_3: return 4;
+    case K_CK_4: return 5;
+    case K_CK_5: return 6;
+    case K_CK_6: return 7;
+    case K_ENERGY: return 1;
+    default: return 0;
+  }
+}
+
+}  // namespace
+
+struct gpe_ctx {
+  gpe_params P{};
+  int R = 1;
+  int prec = GPE_FP64;
+  size_t n = 0;
+  size_t esz = 16;  // bytes per stored complex
+  bool comp = false;
+  hipStream_t stream = nullptr;
+  // fields
+  void *psi = nullptr, *psi2 = nullptr, *lo = nullptr, *lo2 = nullptr;
+  void *k[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  void *acc = nullptr, *scale = nullptr;
+  // reductions
+  double *part = nullptr;  // 3 * part_cap
+  int part_cap = 0;
+  double *dscal = nullptr;   // device scalars: norm, emax, energy
+  double *hscal = nullptr;   // pinned mirror
+  double *hist = nullptr;
+  int64_t hist_cap = 0, hist_len = 0;
+  bool uploaded = false;
+  gpe_step_state st{};
+  StageArgs base{};
+  int zc_override = 0;
+  // profiling
+  bool prof = false;
+  struct Ev {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<Ev> pending;
+  std::vector<hipEvent_t> pool;
+  int64_t prof_n[GPE_PROF_KINDS] = {};
+  double prof_ms[GPE_PROF_KINDS] = {};
+};
+
+namespace {
+
+int grid_for(gpe_ctx *c, int kind, StageArgs &A) {
+  Tile t = tile_shape((int)c->esz == 16 ? 16 : 8, c->R, kind);
+  const int TY = t.nw * t.p;
+  A.ntx = (c->P.nx + 63) / 64;
+  A.nty = (c->P.ny + TY - 1) / TY;
+  const int tiles2d = A.ntx * A.nty;
+  int zc = c->zc_override;
+  if (zc <= 0) {
+    // enough workgroups to fill 256 CUs several times over, but z-chunks
+    // long enough that re-reading 2R planes per chunk stays small
+    int nzc = (2048 + tiles2d - 1) / tiles2d;
+    zc = (c->P.nz + nzc - 1) / nzc;
+    if (zc < 32) zc = 32;
+  }
+  if (zc > c->P.nz) zc = c->P.nz;
+  A.zc = zc;
+  A.nzc = (c->P.nz + zc - 1) / zc;
+  A.nblocks = tiles2d * A.nzc;
+  return A.nblocks;
+}
+
+hipEvent_t take_event(gpe_ctx *c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void prof_drain(gpe_ctx *c) {
+  for (auto &ev : c->pending) {
+    float ms = 0.f;
+    hipEventSynchronize(ev.b);
+    if (hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess && ev.kind >= 0 && ev.kind < GPE_PROF_KINDS) {
+      c->prof_n[ev.kind] += 1;
+      c->prof_ms[ev.kind] += ms;
+    }
+    c->pool.push_back(ev.a);
+    c->pool.push_back(ev.b);
+  }
+  c->pending.clear();
+}
+
+int run_kind(gpe_ctx *c, int kind, StageArgs &A) {
+  grid_for(c, kind, A);
+  if (A.nblocks > c->part_cap) return fail(GPE_EINVAL, "internal: %d workgroups exceed partial buffer", A.nblocks);
+  hipEvent_t ea = nullptr, eb = nullptr;
+  if (c->prof) {
+    ea = take_event(c);
+    eb = take_event(c);
+    if (ea) hipEventRecord(ea, c->stream);
+  }
+  hipError_t e;
+  if (c->prec == GPE_FP64)
+    e = launch_stage_f64(kind, c->R, A, c->stream);
+  else
+    e = launch_stage_f32(kind, c->R, c->comp, A, c->stream);
+  if (e != hipSuccess) return fail(GPE_EHIP, "stage %s launch: %s", kind_name(kind), hipGetErrorString(e));
+  if (c->prof && ea && eb) {
+    hipEventRecord(eb, c->stream);
+    c->pending.push_back({kind, ea, eb});
+    if (c->pending.size() > 4096) prof_drain(c);
+  }
+  return GPE_OK;
+}
+
+StageArgs args_for(gpe_ctx *c) {
+  StageArgs A = c->base;
+  A.dtr = c->st.dt_re;
+  A.dti = c->st.dt_im;
+  A.hr = 0.5 * c->st.dt_re;
+  A.hi = 0.5 * c->st.dt_im;
+  A.part_norm = c->part;
+  A.part_emax = c->part + c->part_cap;
+  A.part_energy = c->part + 2 * c->part_cap;
+  return A;
+}
+
+int reduce_after(gpe_ctx *c, const StageArgs &A, bool emax, bool energy, double *hist_slot) {
+  const double dv = c->P.dx * c->P.dy * c->P.dz;
+  hipError_t e = launch_reduce(A.part_norm, emax ? A.part_emax : nullptr, energy ? A.part_energy : nullptr,
+                               A.nblocks, dv, c->P.eps, dv, c->dscal, hist_slot, c->stream);
+  if (e != hipSuccess) return fail(GPE_EHIP, "reduce launch: %s", hipGetErrorString(e));
+  return GPE_OK;
+}
+
+void swap_state(gpe_ctx *c) {
+  std::swap(c->psi, c->psi2);
+  if (c->comp) std::swap(c->lo, c->lo2);
+}
+
+// One solver() call on the resident field.
+int step_once(gpe_ctx *c, double *hist_slot) {
+  int rc;
+  const int sch = c->P.scheme;
+  if (sch == GPE_EULER) {
+    StageArgs A = args_for(c);
+    A.psi = c->psi;
+    A.lo = c->lo;
+    A.psi_out = c->psi2;
+    A.lo_out = c->lo2;
+    if ((rc = run_kind(c, K_EULER, A))) return rc;
+    if ((rc = reduce_after(c, A, false, false, hist_slot))) return rc;
+    swap_state(c);
+  } else if (sch == GPE_RK2) {
+    StageArgs A = args_for(c);
+    A.psi = c->psi;
+    A.kout = c->k[0];
+    if ((rc = run_kind(c, K_RK2_1, A))) return rc;
+    A = args_for(c);
+    A.psi = c->psi;
+    A.lo = c->lo;
+    A.kin[0] = c->k[0];
+    A.psi_out = c->psi2;
+    A.lo_out = c->lo2;
+    if ((rc = run_kind(c, K_RK2_2, A))) return rc;
+    if ((rc = reduce_after(c, A, false, false, hist_slot))) return rc;
+    swap_state(c);
+  } else if (sch == GPE_RK4) {
+    // k[0]: k1 then k3 (k1 is folded into acc by stage 2), k[1]: k2
+    StageArgs A = args_for(c);
+    A.psi = c->psi;
+    A.kout = c->k[0];
+    if ((rc = run_kind(c, K_RK4_1, A))) return rc;
+    A = args_for(c);
+    A.psi = c->psi;
+    A.kin[0] = c->k[0];
+    A.kout = c->k[1];
+    A.acc_out = c->acc;
+    if ((rc = run_kind(c, K_RK4_2, A))) return rc;
+    A = args_for(c);
+    A.psi = c->psi;
+    A.kin[0] = c->k[1];
+    A.kout = c->k[0];
+    if ((rc = run_kind(c, K_RK4_3, A))) return rc;
+    A = args_for(c);
+    A.psi = c->psi;
+    A.lo = c->lo;
+    A.kin[0] = c->k[0];
+    A.acc_in = c->acc;
+    A.psi_out = c->psi2;
+    A.lo_out = c->lo2;
+    if ((rc = run_kind(c, K_RK4_4, A))) return rc;
+    if ((rc = reduce_after(c, A, false, false, hist_slot))) return rc;
+    swap_state(c);
+  } else {
+    return GPE_EINVAL;  // rk45 is driven by step_rk45
+  }
+  const double dtr = c->st.dt_re, dti = c->st.dt_im;
+  if (c->prec == GPE_FP32) {
+    c->st.t = (double)((float)c->st.t + (float)dtr);
+    c->st.im_t = (double)((float)c->st.im_t + std::fabs((float)dti));
+  } else {
+    c->st.t = c->st.t + dtr;
+    c->st.im_t = c->st.im_t + std::fabs(dti);
+  }
+  return GPE_OK;
+}
+
+// rkqs (solve.f90:158-244) with the arithmetic of the storage precision for
+// the plain FP32 path (the reference built with pr = single) and FP64
+// otherwise.
+template <typename RT>
+struct Ctl {
+  static RT pw(RT a, RT b) { return (RT)std::pow(a, b); }
+};
+
+template <typename RT>
+int rkqs(gpe_ctx *c, double *hist_slot) {
+  const RT safety = (RT)c->P.safety, eps = (RT)c->P.eps;
+  const RT dt_inc = (RT)c->P.dt_increase, dt_dec = (RT)c->P.dt_decrease;
+  const RT errcon = Ctl<RT>::pw((RT)5.0 / safety, (RT)1.0 / dt_inc);
+  RT dtr = (RT)c->st.dt_re, dti = (RT)c->st.dt_im;
+  const double sdtr = c->st.dt_re, sdti = c->st.dt_im;
+  RT errmax = 0;
+  int rejects = 0;
+  int rc;
+  for (int attempt = 0;; ++attempt) {
+    c->st.dt_re = (double)dtr;
+    c->st.dt_im = (double)dti;
+    for (int s = 1; s <= 6; ++s) {
+      StageArgs A = args_for(c);
+      A.sdtr = sdtr;
+      A.sdti = sdti;
+      A.psi = c->psi;
+      A.lo = c->lo;
+      for (int m = 0; m < s - 1; ++m) A.kin[m] = c->k[m];
+      static const double bt[7][5] = {{0, 0, 0, 0, 0},
+                                      {0, 0, 0, 0, 0},
+                                      {0.2, 0, 0, 0, 0},
+                                      {0.075, 0.225, 0, 0, 0},
+                                      {0.3, -0.9, 1.2, 0, 0},
+                                      {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0},
+                                      {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0,
+                                       253.0 / 4096.0}};
+      for (int m = 0; m < 5; ++m) A.b[m] = bt[s][m];
+      if (s < 6) {
+        A.kout = c->k[s - 1];
+        if (s == 1) {
+          A.write_scale = attempt == 0;
+          A.scale_out = c->scale;
+        }
+        if ((rc = run_kind(c, K_CK_1 + s - 1, A))) return rc;
+      } else {
+        A.scale_in = c->scale;
+        A.psi_out = c->psi2;
+        A.lo_out = c->lo2;
+        if ((rc = run_kind(c, K_CK_6, A))) return rc;
+        if ((rc = reduce_after(c, A, true, false, hist_slot))) return rc;
+      }
+    }
+    HIPCHK(hipMemcpyAsync(c->hscal, c->dscal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double em = c->hscal[1];  // maxval(|err/psi_scal|)/eps
+    errmax = (RT)em;
+    if (!(errmax == errmax)) return fail(GPE_ENONFINITE, "rk45: NaN in the error norm (t=%g)", c->st.t);
+    if (errmax <= (RT)1.0) break;
+    if (++rejects > 1000) return fail(GPE_ESTEP, "rk45: more than 1000 rejected attempts (t=%g)", c->st.t);
+    const RT pwv = Ctl<RT>::pw(errmax, dt_dec);
+    RT ar = safety * dtr, ai = safety * dti;
+    ar = ar * pwv;
+    ai = ai * pwv;
+    const RT a = (RT)std::hypot(ar, ai), b = (RT)0.1 * (RT)std::hypot(dtr, dti);
+    const RT mx = a > b ? a : b;
+    if (!c->P.real_time) {
+      dtr = (RT)0.0;
+      dti = -mx;
+    } else {
+      dtr = mx;
+      dti = (RT)0.0;
+    }
+  }
+  RT nr, ni;
+  if (errmax > errcon) {
+    const RT pwv = Ctl<RT>::pw(errmax, dt_inc);
+    RT ar = safety * dtr, ai = safety * dti;
+    nr = ar * pwv;
+    ni = ai * pwv;
+  } else {
+    nr = (RT)5.0 * dtr;
+    ni = (RT)5.0 * dti;
+  }
+  c->st.t = (double)((RT)c->st.t + dtr);
+  c->st.dt_re = (double)nr;
+  c->st.dt_im = (double)ni;
+  c->st.im_t = (double)((RT)c->st.im_t + (RT)std::fabs(ni));
+  c->st.errmax = (double)errmax;
+  c->st.n_rejects = rejects;
+  swap_state(c);
+  return GPE_OK;
+}
+
+int solver_call(gpe_ctx *c, double *hist_slot) {
+  if (c->P.scheme == GPE_RK45) return c->prec == GPE_FP32 ? rkqs<float>(c, hist_slot) : rkqs<double>(c, hist_slot);
+  return step_once(c, hist_slot);
+}
+
+void free_all(gpe_ctx *c) {
+  void *ptrs[] = {c->psi, c->psi2, c->lo, c->lo2, c->k[0], c->k[1], c->k[2], c->k[3], c->k[4], c->k[5],
+                  c->acc, c->scale, c->part, c->dscal, c->hist};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  if (c->hscal) hipHostFree(c->hscal);
+  for (auto &ev : c->pending) {
+    hipEventDestroy(ev.a);
+    hipEventDestroy(ev.b);
+  }
+  for (auto e : c->pool) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+}
+
+int check_params(const gpe_params *p) {
+  if (!p) return fail(GPE_EINVAL, "params is NULL");
+  if (p->nx < 5 || p->ny < 5 || p->nz < 5) return fail(GPE_EINVAL, "grid must be at least 5 points per axis");
+  if (p->scheme != GPE_EULER && p->scheme != GPE_RK2 && p->scheme != GPE_RK4 && p->scheme != GPE_RK45)
+    return fail(GPE_EINVAL, "ERROR: Unrecognised time stepping scheme.");
+  if (p->eqn < 1 || p->eqn > 4) return fail(GPE_EINVAL, "eqn_to_solve must be 1..4");
+  if (p->bcs != 1 && p->bcs != 2) return fail(GPE_EINVAL, "bcs must be 1 (periodic) or 2 (reflective)");
+  if (p->order != 2 && p->order != 4) return fail(GPE_EINVAL, "order must be 2 or 4");
+  if (p->precision < GPE_FP64 || p->precision > GPE_FP32C) return fail(GPE_EINVAL, "unknown precision");
+  if (!(p->dx > 0 && p->dy > 0 && p->dz > 0)) return fail(GPE_EINVAL, "grid spacing must be positive");
+  return GPE_OK;
+}
+
+int alloc_field(gpe_ctx *c, void **p) {
+  hipError_t e = hipMalloc(p, c->n * c->esz);
+  if (e != hipSuccess) return fail(GPE_ENOMEM, "hipMalloc of %zu bytes: %s", c->n * c->esz, hipGetErrorString(e));
+  return GPE_OK;
+}
+
+// Host <-> device conversions for uploads/downloads.
+int to_device(gpe_ctx *c, const void *src, int host_type, size_t src_pitch_planes, size_t row_off) {
+  // src_pitch_planes: elements between z-planes in the host array; row_off:
+  // element offset of (0, 0, 0) of the interior.
+  const size_t plane = (size_t)c->P.nx * c->P.ny;
+  const bool contiguous = src_pitch_planes == plane && row_off == 0;
+  if (c->prec == GPE_FP64 && host_type == GPE_HOST_C128) {
+    if (contiguous) {
+      HIPCHK(hipMemcpy(c->psi, src, c->n * 16, hipMemcpyHostToDevice));
+    } else {
+      HIPCHK(hipMemcpy2D(c->psi, plane * 16, (const char *)src + row_off * 16, src_pitch_planes * 16, plane * 16,
+                         c->P.nz, hipMemcpyHostToDevice));
+    }
+    return GPE_OK;
+  }
+  if (c->prec == GPE_FP32 && host_type == GPE_HOST_C64) {
+    if (contiguous) {
+      HIPCHK(hipMemcpy(c->psi, src, c->n * 8, hipMemcpyHostToDevice));
+    } else {
+      HIPCHK(hipMemcpy2D(c->psi, plane * 8, (const char *)src + row_off * 8, src_pitch_planes * 8, plane * 8,
+                         c->P.nz, hipMemcpyHostToDevice));
+    }
+    return GPE_OK;
+  }
+  if (c->prec == GPE_FP64) return fail(GPE_EINVAL, "FP64 context takes complex128 host data");
+  // FP32 / FP32C from complex128: split into hi (and lo) on the host.
+  if (host_type != GPE_HOST_C128 && c->comp) {
+    // complex64 into a compensated context: lo = 0
+    std::vector<float> hi(2 * plane);
+    for (int k = 0; k < c->P.nz; ++k) {
+      const float *s = (const float *)src + 2 * (row_off + (size_t)k * src_pitch_planes);
+      HIPCHK(hipMemcpy((char *)c->psi + (size_t)k * plane * 8, s, plane * 8, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemset(c->lo, 0, c->n * 8));
+    return GPE_OK;
+  }
+  if (host_type != GPE_HOST_C128) return fail(GPE_EINVAL, "unsupported host type");
+  std::vector<float> hi(2 * plane), lo(2 * plane);
+  for (int k = 0; k < c->P.nz; ++k) {
+    const double *s = (const double *)src + 2 * (row_off + (size_t)k * src_pitch_planes);
+    for (size_t q = 0; q < 2 * plane; ++q) {
+      float h = (float)s[q];
+      hi[q] = h;
+      lo[q] = (float)(s[q] - (double)h);
+    }
+    HIPCHK(hipMemcpy((char *)c->psi + (size_t)k * plane * 8, hi.data(), plane * 8, hipMemcpyHostToDevice));
+    if (c->comp)
+      HIPCHK(hipMemcpy((char *)c->lo + (size_t)k * plane * 8, lo.data(), plane * 8, hipMemcpyHostToDevice));
+  }
+  return GPE_OK;
+}
+
+int from_device(gpe_ctx *c, void *dst, int host_type) {
+  const size_t plane = (size_t)c->P.nx * c->P.ny;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->prec == GPE_FP64) {
+    if (host_type != GPE_HOST_C128) return fail(GPE_EINVAL, "FP64 context returns complex128 host data");
+    HIPCHK(hipMemcpy(dst, c->psi, c->n * 16, hipMemcpyDeviceToHost));
+    return GPE_OK;
+  }
+  if (host_type == GPE_HOST_C64 && !c->comp) {
+    HIPCHK(hipMemcpy(dst, c->psi, c->n * 8, hipMemcpyDeviceToHost));
+    return GPE_OK;
+  }
+  std::vector<float> hi(2 * plane), lo(2 * plane, 0.f);
+  for (int k = 0; k < c->P.nz; ++k) {
+    HIPCHK(hipMemcpy(hi.data(), (char *)c->psi + (size_t)k * plane * 8, plane * 8, hipMemcpyDeviceToHost));
+    if (c->comp)
+      HIPCHK(hipMemcpy(lo.data(), (char *)c->lo + (size_t)k * plane * 8, plane * 8, hipMemcpyDeviceToHost));
+    if (host_type == GPE_HOST_C128) {
+      double *d = (double *)dst + 2 * (size_t)k * plane;
+      for (size_t q = 0; q < 2 * plane; ++q) d[q] = (double)hi[q] + (double)lo[q];
+    } else {
+      float *d = (float *)dst + 2 * (size_t)k * plane;
+      for (size_t q = 0; q < 2 * plane; ++q) d[q] = (float)((double)hi[q] + (double)lo[q]);
+    }
+  }
+  return GPE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gpe_last_error(void) { return g_err.c_str(); }
+
+int gpe_ctx_create(const gpe_params *params, gpe_ctx **out) {
+  if (!out) return fail(GPE_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = check_params(params);
+  if (rc) return rc;
+  gpe_ctx *c = new (std::nothrow) gpe_ctx();
+  if (!c) return fail(GPE_ENOMEM, "host allocation failed");
+  c->P = *params;
+  c->R = params->order / 2;
+  c->prec = params->precision;
+  c->comp = params->precision == GPE_FP32C;
+  c->esz = params->precision == GPE_FP64 ? 16 : 8;
+  c->n = (size_t)params->nx * params->ny * params->nz;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    rc = fail(GPE_EHIP, "hipStreamCreate failed");
+    delete c;
+    return rc;
+  }
+  int nk = 0;
+  switch (params->scheme) {
+    case GPE_EULER: nk = 0; break;
+    case GPE_RK2: nk = 1; break;
+    case GPE_RK4: nk = 2; break;
+    case GPE_RK45: nk = 5; break;
+  }
+  if ((rc = alloc_field(c, &c->psi)) || (rc = alloc_field(c, &c->psi2))) goto bad;
+  if (c->comp && ((rc = alloc_field(c, &c->lo)) || (rc = alloc_field(c, &c->lo2)))) goto bad;
+  for (int m = 0; m < nk; ++m)
+    if ((rc = alloc_field(c, &c->k[m]))) goto bad;
+  if (params->scheme == GPE_RK4 && (rc = alloc_field(c, &c->acc))) goto bad;
+  if (params->scheme == GPE_RK45 && (rc = alloc_field(c, &c->scale))) goto bad;
+  {
+    // partial buffers sized for the largest grid of any kind
+    int cap = 0;
+    for (int kk = 0; kk < K_COUNT; ++kk) {
+      StageArgs tmp{};
+      int nb = grid_for(c, kk, tmp);
+      if (nb > cap) cap = nb;
+    }
+    if (cap < PW_BLOCKS) cap = PW_BLOCKS;
+    c->part_cap = cap;
+    if (hipMalloc((void **)&c->part, 3 * sizeof(double) * (size_t)cap) != hipSuccess) {
+      rc = fail(GPE_ENOMEM, "partials");
+      goto bad;
+    }
+    if (hipMalloc((void **)&c->dscal, 4 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void **)&c->hscal, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      rc = fail(GPE_ENOMEM, "scalars");
+      goto bad;
+    }
+  }
+  {
+    StageArgs &A = c->base;
+    A.nx = params->nx;
+    A.ny = params->ny;
+    A.nz = params->nz;
+    A.periodic = params->bcs == 1;
+    A.eqn = params->eqn;
+    A.use_diss = params->diss_amp != 0.0;
+    A.use_urhs = params->urhs != 0.0;
+    A.xl = params->xl;
+    A.yl = params->yl;
+    A.zl = params->zl;
+    A.dx = params->dx;
+    A.dy = params->dy;
+    A.dz = params->dz;
+    A.g = params->g;
+    A.mu = params->mu;
+    A.ox2 = params->omx * params->omx;
+    A.oy2 = params->omy * params->omy;
+    A.oz2 = params->omz * params->omz;
+    A.urhs = params->urhs;
+    A.diss_amp = params->diss_amp;
+    // a = x(nx-3) etc. (solve.f90:397-399)
+    A.a_diss = params->xl + (double)(params->nx - 3) * params->dx;
+    A.b_diss = params->yl + (double)(params->ny - 3) * params->dy;
+    A.c_diss = params->zl + (double)(params->nz - 3) * params->dz;
+    const double dx2 = params->dx * params->dx, dy2 = params->dy * params->dy, dz2 = params->dz * params->dz;
+    const double sxx = c->R == 1 ? dx2 : 12.0 * dx2;
+    const double syy = c->R == 1 ? dy2 : 12.0 * dy2;
+    const double szz = c->R == 1 ? dz2 : 12.0 * dz2;
+    const double sx = c->R == 1 ? 2.0 * params->dx : 12.0 * params->dx;
+    if (c->prec == GPE_FP32) {
+      // the reference built with pr = single forms these in single precision
+      const float fdx = (float)params->dx, fdy = (float)params->dy, fdz = (float)params->dz;
+      const float fx2 = fdx * fdx, fy2 = fdy * fdy, fz2 = fdz * fdz;
+      A.inv_xx = 1.0 / (double)(c->R == 1 ? fx2 : 12.0f * fx2);
+      A.inv_yy = 1.0 / (double)(c->R == 1 ? fy2 : 12.0f * fy2);
+      A.inv_zz = 1.0 / (double)(c->R == 1 ? fz2 : 12.0f * fz2);
+      A.inv_x = 1.0 / (double)(c->R == 1 ? 2.0f * fdx : 12.0f * fdx);
+    } else {
+      A.inv_xx = 1.0 / sxx;
+      A.inv_yy = 1.0 / syy;
+      A.inv_zz = 1.0 / szz;
+      A.inv_x = 1.0 / sx;
+    }
+    A.d_xx = make_divc(sxx);
+    A.d_yy = make_divc(syy);
+    A.d_zz = make_divc(szz);
+    A.d_x = make_divc(sx);
+    A.d6 = make_divc(6.0);
+    A.d3 = make_divc(3.0);
+    // Cash-Karp weights (solve.f90:270-281), formed in double as the
+    // reference forms its real(pr) parameters
+    const double c1 = 37.0 / 378.0, c2 = 0.0, c3 = 250.0 / 621.0, c4 = 125.0 / 594.0, c5 = 0.0,
+                 c6 = 512.0 / 1771.0;
+    A.c[1] = c1;
+    A.c[2] = c2;
+    A.c[3] = c3;
+    A.c[4] = c4;
+    A.c[5] = c5;
+    A.c[6] = c6;
+    A.dc[1] = c1 - 2825.0 / 27648.0;
+    A.dc[2] = c2 - 0.0;
+    A.dc[3] = c3 - 18575.0 / 48384.0;
+    A.dc[4] = c4 - 13525.0 / 55296.0;
+    A.dc[5] = c5 - 277.0 / 14336.0;
+    A.dc[6] = c6 - 0.25;
+    A.w1 = 3.0 / 8.0;
+    A.w2 = 7.0 / 6.0;
+    A.w3 = 23.0 / 24.0;
+  }
+  *out = c;
+  return GPE_OK;
+bad:
+  free_all(c);
+  delete c;
+  return rc;
+}
+
+void gpe_ctx_destroy(gpe_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  free_all(ctx);
+  delete ctx;
+}
+
+int gpe_sim_upload(gpe_ctx *c, const void *psi, int32_t host_type) {
+  if (!c || !psi) return fail(GPE_EINVAL, "NULL argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t plane = (size_t)c->P.nx * c->P.ny;
+  int rc = to_device(c, psi, host_type, plane, 0);
+  if (rc) return rc;
+  if (c->comp && host_type != GPE_HOST_C128 && host_type != GPE_HOST_C64) return fail(GPE_EINVAL, "host type");
+  c->uploaded = true;
+  c->hist_len = 0;
+  return GPE_OK;
+}
+
+int gpe_sim_download(gpe_ctx *c, void *psi, int32_t host_type) {
+  if (!c || !psi) return fail(GPE_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  return from_device(c, psi, host_type);
+}
+
+int gpe_sim_set_state(gpe_ctx *c, const gpe_step_state *st) {
+  if (!c || !st) return fail(GPE_EINVAL, "NULL argument");
+  c->st = *st;
+  return GPE_OK;
+}
+
+int gpe_sim_get_state(gpe_ctx *c, gpe_step_state *st) {
+  if (!c || !st) return fail(GPE_EINVAL, "NULL argument");
+  *st = c->st;
+  return GPE_OK;
+}
+
+int gpe_sim_advance(gpe_ctx *c, int64_t nsteps, int32_t record_norm) {
+  if (!c) return fail(GPE_EINVAL, "NULL context");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  if (nsteps < 0) return fail(GPE_EINVAL, "nsteps < 0");
+  if (record_norm) {
+    if (c->hist_cap < nsteps) {
+      if (c->hist) hipFree(c->hist);
+      c->hist = nullptr;
+      HIPCHK(hipMalloc((void **)&c->hist, sizeof(double) * (size_t)(nsteps > 0 ? nsteps : 1)));
+      c->hist_cap = nsteps;
+    }
+    c->hist_len = 0;
+  }
+  for (int64_t s = 0; s < nsteps; ++s) {
+    double *slot = record_norm ? c->hist + s : nullptr;
+    int rc = solver_call(c, slot);
+    if (rc) return rc;
+    c->st.p += 1;
+    if (record_norm) c->hist_len = s + 1;
+  }
+  if (c->prof) prof_drain(c);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GPE_OK;
+}
+
+int gpe_sim_norm_history(gpe_ctx *c, double *out, int64_t n) {
+  if (!c || !out) return fail(GPE_EINVAL, "NULL argument");
+  if (n > c->hist_len) return fail(GPE_EINVAL, "only %lld norms recorded", (long long)c->hist_len);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (n > 0) HIPCHK(hipMemcpy(out, c->hist, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+  return GPE_OK;
+}
+
+static int norm_like(gpe_ctx *c, int weighted, double *out) {
+  if (!c || !out) return fail(GPE_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  hipError_t e;
+  const StageArgs &A = c->base;
+  if (c->prec == GPE_FP64)
+    e = launch_norm_f64(c->psi, A.nx, A.ny, A.nz, weighted, A.w1, A.w2, A.w3, c->part, c->stream);
+  else
+    e = launch_norm_f32(c->psi, c->comp ? c->lo : nullptr, A.nx, A.ny, A.nz, weighted, A.w1, A.w2, A.w3, c->part,
+                        c->stream);
+  if (e != hipSuccess) return fail(GPE_EHIP, "norm launch: %s", hipGetErrorString(e));
+  const double dv = c->P.dx * c->P.dy * c->P.dz;
+  e = launch_reduce(c->part, nullptr, nullptr, PW_BLOCKS, dv, c->P.eps, dv, c->dscal, nullptr, c->stream);
+  if (e != hipSuccess) return fail(GPE_EHIP, "reduce launch: %s", hipGetErrorString(e));
+  HIPCHK(hipMemcpyAsync(c->hscal, c->dscal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *out = c->hscal[0];
+  return GPE_OK;
+}
+
+int gpe_sim_norm(gpe_ctx *c, double *norm) { return norm_like(c, 1, norm); }
+int gpe_sim_mass(gpe_ctx *c, double *mass) { return norm_like(c, 0, mass); }
+
+int gpe_sim_energy(gpe_ctx *c, double *energy) {
+  if (!c || !energy) return fail(GPE_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  StageArgs A = args_for(c);
+  A.psi = c->psi;
+  int rc = run_kind(c, K_ENERGY, A);
+  if (rc) return rc;
+  if ((rc = reduce_after(c, A, false, true, nullptr))) return rc;
+  HIPCHK(hipMemcpyAsync(c->hscal, c->dscal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *energy = c->hscal[2];
+  return GPE_OK;
+}
+
+int gpe_sim_renormalise(gpe_ctx *c, double nn, double norm) {
+  if (!c) return fail(GPE_EINVAL, "NULL context");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  hipError_t e;
+  if (c->prec == GPE_FP64)
+    e = launch_renorm_f64(c->psi, c->n, std::sqrt(nn), make_divc(std::sqrt(norm)), c->stream);
+  else
+    e = launch_renorm_f32(c->psi, c->comp ? c->lo : nullptr, c->n, std::sqrt(nn), std::sqrt(norm), c->stream);
+  if (e != hipSuccess) return fail(GPE_EHIP, "renormalise launch: %s", hipGetErrorString(e));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GPE_OK;
+}
+
+int gpe_sim_rhs(gpe_ctx *c, void *out, int32_t host_type) {
+  if (!c || !out) return fail(GPE_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  void *tmp = c->psi2;  // scratch: the next-state buffer is free between steps
+  StageArgs A = args_for(c);
+  A.psi = c->psi;
+  A.kout = tmp;
+  int rc = run_kind(c, K_RHS, A);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->prec == GPE_FP64) {
+    if (host_type != GPE_HOST_C128) return fail(GPE_EINVAL, "FP64 context returns complex128 host data");
+    HIPCHK(hipMemcpy(out, tmp, c->n * 16, hipMemcpyDeviceToHost));
+  } else {
+    if (host_type == GPE_HOST_C64) {
+      HIPCHK(hipMemcpy(out, tmp, c->n * 8, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> f(2 * c->n);
+      HIPCHK(hipMemcpy(f.data(), tmp, c->n * 8, hipMemcpyDeviceToHost));
+      double *d = (double *)out;
+      for (size_t q = 0; q < 2 * c->n; ++q) d[q] = f[q];
+    }
+  }
+  return GPE_OK;
+}
+
+int gpe_solver_step(gpe_ctx *c, const void *var_in, int32_t halo, void *var_out, gpe_step_state *st) {
+  if (!c || !var_in || !var_out || !st) return fail(GPE_EINVAL, "NULL argument");
+  if (halo < 0) return fail(GPE_EINVAL, "halo < 0");
+  if (c->prec == GPE_FP32C) return fail(GPE_EINVAL, "the compensated form is device-resident only");
+  const int ht = c->prec == GPE_FP64 ? GPE_HOST_C128 : GPE_HOST_C64;
+  const size_t nx = c->P.nx, nyh = (size_t)c->P.ny + 2 * halo;
+  const size_t pitch = nx * nyh;                       // elements per host z-plane
+  const size_t off = (size_t)halo * nx + (size_t)halo * pitch;  // (0, js, ks)
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc = to_device(c, var_in, ht, pitch, off);
+  if (rc) return rc;
+  c->uploaded = true;
+  c->st = *st;
+  rc = solver_call(c, nullptr);
+  if (rc) return rc;
+  rc = from_device(c, var_out, ht);
+  if (rc) return rc;
+  gpe_step_state o = c->st;
+  o.p = st->p;
+  *st = o;
+  if (c->prof) prof_drain(c);
+  return GPE_OK;
+}
+
+int gpe_prof_enable(gpe_ctx *c, int32_t enable) {
+  if (!c) return fail(GPE_EINVAL, "NULL context");
+  if (c->prof) prof_drain(c);
+  c->prof = enable != 0;
+  for (int k = 0; k < GPE_PROF_KINDS; ++k) {
+    c->prof_n[k] = 0;
+    c->prof_ms[k] = 0.0;
+  }
+  return GPE_OK;
+}
+
+int gpe_prof_query(gpe_ctx *c, int32_t kind, int64_t *launches, double *total_ms, double *bytes_per_launch,
+                   char *name) {
+  if (!c) return fail(GPE_EINVAL, "NULL context");
+  if (kind < 0 || kind >= K_COUNT) return fail(GPE_EINVAL, "kind out of range");
+  prof_drain(c);
+  if (launches) *launches = c->prof_n[kind];
+  if (total_ms) *total_ms = c->prof_ms[kind];
+  if (bytes_per_launch) {
+    double passes = kind_passes(kind);
+    double bytes = passes * (double)c->esz;
+    if (c->comp && (kind == K_EULER || kind == K_RK2_2 || kind == K_RK4_4 || kind == K_CK_6)) bytes += 2.0 * c->esz;
+    *bytes_per_launch = bytes * (double)c->n;
+  }
+  if (name) {
+    std::snprintf(name, 64, "%s", kind_name(kind));
+  }
+  return GPE_OK;
+}
+
+}  // extern "C"

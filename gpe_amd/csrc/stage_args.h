@@ -1,0 +1,82 @@
+// stage_args.h -- POD launch descriptor shared by the host code and the
+// stage kernels (passed by value as the kernel argument).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+namespace gpe {
+
+// One fused sweep of the time-stepping path.  Each kind reads psi and the
+// k-fields its stage input needs, evaluates f(u) on the 2nd/4th-order
+// stencil and writes what the next stage (or the new state) needs.
+enum Kind : int {
+  K_RHS = 0,    // out = f(psi)                       (get_rhs, solve.f90:382)
+  K_EULER,      // psi' = psi + dt f(psi)              (solve.f90:57-74)
+  K_RK2_1,      // k1 = f(psi)                         (solve.f90:91)
+  K_RK2_2,      // psi' = psi + dt f(psi + dt/2 k1)     (solve.f90:92-99)
+  K_RK4_1,      // k1 = f(psi)                         (solve.f90:124)
+  K_RK4_2,      // k2 = f(psi + dt/2 k1); acc = k1/6 + k2/3
+  K_RK4_3,      // k3 = f(psi + dt/2 k2)
+  K_RK4_4,      // psi' = psi + dt((acc + k3/3) + f(psi + dt k3)/6)  (144-148)
+  K_CK_1,       // kk1 = dt f(psi); scale = |psi| + dt|f| + 1e-30 (176-178, 293-296)
+  K_CK_2, K_CK_3, K_CK_4, K_CK_5, // kk_s = dt f(psi + sum b_sm kk_m)
+  K_CK_6,       // kk6; psi' = psi + sum c kk; err = sum dc kk; max|err/scale|
+  K_ENERGY,     // sum Re(-lap conj psi) + 0.5|psi|^4  (variables.f90:580-604)
+  K_COUNT
+};
+
+// Division by a real constant as the reference build evaluates it: the
+// divisor is promoted to complex and divided by compiler-rt's logb-scaled
+// quotient, i.e. ((a*cs + b*0)/den, (b*cs - a*0)/den) * 2^-e with
+// cs = s*2^-e, den = cs*cs + 0*0, e = logb(s).
+struct DivC {
+  double cs, den;
+  int e;
+  int pad;
+};
+
+struct StageArgs {
+  // geometry and tiling
+  int nx, ny, nz;
+  int periodic;  // bcs == 1
+  int eqn;
+  int ntx, nty, nzc, zc;
+  int nblocks;
+  int write_scale;  // K_CK_1 on the first attempt of a step
+  int use_diss;     // diss_amp != 0
+  int use_urhs;     // urhs != 0
+  int pad0;
+  // coordinates and coefficients (ic.f90:32-69, 713-731)
+  double xl, yl, zl, dx, dy, dz;
+  double g, mu, ox2, oy2, oz2, urhs, diss_amp, a_diss, b_diss, c_diss;
+  // FP64 divisions in the reference's form
+  DivC d_xx, d_yy, d_zz;  // dx2 (order 2) or 12*dx2 (order 4)
+  DivC d_x;               // 2*dx or 12*dx
+  DivC d6, d3;            // RK4 combination /6 and /3
+  // FP32 reciprocals
+  double inv_xx, inv_yy, inv_zz, inv_x;
+  // time step (complex dt) and stage coefficients
+  double dtr, dti;   // dt
+  double hr, hi;     // 0.5*dt
+  double sdtr, sdti; // dt at entry of the rk45 step (scale)
+  double b[5];       // Cash-Karp b_{s,1..s-1} of this stage
+  double c[7], dc[7];
+  // norm weights (variables.f90:49-51)
+  double w1, w2, w3;
+  // fields (device pointers; complex interleaved)
+  const void *psi;
+  const void *lo;          // FP32C: low part of psi
+  const void *kin[5];      // input k-fields
+  const void *acc_in;      // K_RK4_4
+  const void *scale_in;    // K_CK_6
+  void *kout;              // k / kk output
+  void *acc_out;           // K_RK4_2
+  void *psi_out;           // new state
+  void *lo_out;            // FP32C new low part
+  void *scale_out;         // K_CK_1
+  double *part_norm;       // per-block partials
+  double *part_emax;
+  double *part_energy;
+};
+
+}  // namespace gpe

@@ -1,0 +1,598 @@
+// stencil.hpp -- the fused stage kernel of the GP time-stepping path (gfx950).
+//
+// One workgroup = NW wave64s owns a TX x TY tile of the xy plane (TX = 64:
+// one x-site per lane, so a row is one coalesced wave access; TY = NW*P:
+// each wave owns P rows) and marches z over a chunk of planes.  For every
+// plane it
+//   * loads the new plane k+R of the stage input u = psi + sum coef*k_m,
+//     reconstructed in registers from psi and the previous stages' k-fields
+//     (so only one field is staged, whatever the stage), into a register
+//     queue of the 2R+1 z-neighbours (R = order/2),
+//   * writes plane k (queue centre) and its x/y ghost cells into a
+//     double-buffered LDS tile,
+//   * evaluates the Laplacian (x and y from LDS, z from the queue), the GP
+//     right-hand side and the stage outputs for plane k,
+//   * keeps the pointwise partial sums the final combination needs in a
+//     small register FIFO (computed when the plane was loaded), so no field
+//     is read twice.
+// Boundaries are index maps applied at load time: periodic wrap (the
+// one-rank halo exchange of variables.f90:203-375 and derivs.f90:286-301)
+// or the mirror about the wall point (derivs.f90:302-313, 365-394).
+//
+// FP64 kernels are compiled with -ffp-contract=off and evaluate every
+// expression in the reference's order with its complex semantics (complex
+// division by a real in compiler-rt's logb-scaled form, abs() as hypot), so
+// they reproduce the reference arithmetic operation by operation.  FP32
+// kernels use reciprocals and FMA; their final psi update and all
+// reductions are FP64 (with an FP32 hi/lo pair when compensated).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "stage_args.h"
+
+namespace gpe {
+
+template <typename T>
+using vec2_t = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
+
+template <typename T>
+struct cx {
+  T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T r, T i) {
+  cx<T> c;
+  c.re = r;
+  c.im = i;
+  return c;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.re + b.re, a.im + b.im); }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return mk(a.re - b.re, a.im - b.im); }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  T r1 = a.re * b.re, r2 = a.im * b.im, i1 = a.re * b.im, i2 = a.im * b.re;
+  return mk(r1 - r2, i1 + i2);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> rmul(T s, cx<T> a) { return mk(s * a.re, s * a.im); }
+template <typename T>
+__device__ __forceinline__ cx<double> todbl(cx<T> a) { return mk((double)a.re, (double)a.im); }
+
+template <typename T>
+__device__ __forceinline__ cx<T> ldc(const void *p, size_t i) {
+  vec2_t<T> v = static_cast<const vec2_t<T> *>(p)[i];
+  return mk<T>(v.x, v.y);
+}
+template <typename T>
+__device__ __forceinline__ void stc(void *p, size_t i, cx<T> a) {
+  vec2_t<T> v;
+  v.x = a.re;
+  v.y = a.im;
+  static_cast<vec2_t<T> *>(p)[i] = v;
+}
+
+// complex / real constant
+__device__ __forceinline__ cx<double> rdiv(cx<double> a, const DivC &d, double) {
+  double x = (a.re * d.cs + a.im * 0.0) / d.den;
+  double y = (a.im * d.cs - a.re * 0.0) / d.den;
+  return mk(ldexp(x, -d.e), ldexp(y, -d.e));
+}
+__device__ __forceinline__ cx<float> rdiv(cx<float> a, const DivC &, double inv) {
+  float f = (float)inv;
+  return mk(a.re * f, a.im * f);
+}
+
+// complex / complex (rk45 error), compiler-rt's logb-scaled quotient
+__device__ __forceinline__ cx<double> cdiv(cx<double> a, cx<double> b) {
+  double c = b.re, d = b.im;
+  double m = fmax(fabs(c), fabs(d));
+  int e = 0;
+  if (m != 0.0 && isfinite(m)) {
+    e = ilogb(m);
+    c = ldexp(c, -e);
+    d = ldexp(d, -e);
+  }
+  double cc = c * c, dd = d * d;
+  double den = cc + dd;
+  double ac = a.re * c, bd = a.im * d, bc = a.im * c, ad = a.re * d;
+  return mk(ldexp((ac + bd) / den, -e), ldexp((bc - ad) / den, -e));
+}
+
+__device__ __forceinline__ double cabs_(cx<double> a) { return hypot(a.re, a.im); }
+__device__ __forceinline__ float cabs_(cx<float> a) { return sqrtf(a.re * a.re + a.im * a.im); }
+
+// Index map for a ghost index i in [-R, n-1+R].
+__device__ __forceinline__ int bcmap(int i, int n, int periodic) {
+  if (periodic) return i < 0 ? i + n : (i >= n ? i - n : i);
+  return i < 0 ? -i : (i >= n ? 2 * (n - 1) - i : i);
+}
+
+// Extended closed-rule weight of index i on an axis of n points
+// (integrate_x/y/z, variables.f90:742-748, 773-778, 818-823).
+__device__ __forceinline__ double nweight(int i, int n, double w1, double w2, double w3) {
+  if (i == 0 || i == n - 1) return w1;
+  if (i == 1 || i == n - 2) return w2;
+  if (i == 2 || i == n - 3) return w3;
+  return 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// Laplacian in the reference's evaluation order (derivs.f90:164-181,
+// 209-224, 251-266; variables.f90:68).
+template <typename T, int R>
+__device__ __forceinline__ cx<T> d2(cx<T> m2, cx<T> m1, cx<T> c, cx<T> p1, cx<T> p2, const DivC &d,
+                                    double inv) {
+  if constexpr (R == 1) {
+    cx<T> num = (p1 - rmul((T)2.0, c)) + m1;
+    return rdiv(num, d, inv);
+  } else {
+    cx<T> num = (((mk(-p2.re, -p2.im) + rmul((T)16.0, p1)) - rmul((T)30.0, c)) + rmul((T)16.0, m1)) - m2;
+    return rdiv(num, d, inv);
+  }
+}
+
+// deriv_x (derivs.f90:41-57)
+template <typename T, int R>
+__device__ __forceinline__ cx<T> d1(cx<T> m2, cx<T> m1, cx<T> p1, cx<T> p2, const DivC &d, double inv) {
+  if constexpr (R == 1) {
+    return rdiv(p1 - m1, d, inv);
+  } else {
+    cx<T> num = ((mk(-p2.re, -p2.im) + rmul((T)8.0, p1)) - rmul((T)8.0, m1)) + m2;
+    return rdiv(num, d, inv);
+  }
+}
+
+// get_rhs at one site (solve.f90:414-437).
+template <typename T>
+__device__ __forceinline__ cx<T> rhs_site(const StageArgs &A, cx<T> u, cx<T> lap, cx<T> dpx, int i, int j,
+                                          int k) {
+  const T x = (T)A.xl + (T)i * (T)A.dx;
+  const T y = (T)A.yl + (T)j * (T)A.dy;
+  const T z = (T)A.zl + (T)k * (T)A.dz;
+  if (A.eqn == 4) {
+    T V = (T)0.5 * (((T)A.ox2 * (x * x) + (T)A.oy2 * (y * y)) + (T)A.oz2 * (z * z));
+    T s;
+    if constexpr (std::is_same<T, double>::value) {
+      cx<T> u2 = cmul(u, u);  // abs(in_var**2)
+      s = cabs_(u2);
+    } else {
+      s = u.re * u.re + u.im * u.im;
+    }
+    cx<T> t1 = rmul((T)-0.5, lap);
+    cx<T> t2 = rmul(V, u);
+    cx<T> t3 = rmul((T)A.g, rmul(s, u));
+    cx<T> t4 = rmul((T)A.mu, u);
+    cx<T> in = ((t1 + t2) + t3) - t4;
+    return mk(in.im, -in.re);  // -eye*(...)
+  }
+  T m = cabs_(u);
+  T m2 = m * m;
+  cx<T> r;
+  if (A.eqn == 2) {
+    cx<T> X = lap - rmul(m2, u);
+    r = mk(-X.im, X.re);  // eye*(...)
+  } else {
+    T diss = (T)0.0;
+    if (A.use_diss) {
+      T tx = (T)1.0 + tanh(x - (T)A.a_diss) * tanh(x + (T)A.a_diss);
+      T ty = (T)1.0 + tanh(y - (T)A.b_diss) * tanh(y + (T)A.b_diss);
+      T tz = (T)1.0 + tanh(z - (T)A.c_diss) * tanh(z + (T)A.c_diss);
+      diss = (T)A.diss_amp * ((tx + ty) + tz);
+    }
+    cx<T> X = lap + rmul((T)1.0 - m2, u);
+    r = cmul(mk((T)0.5 * diss, (T)0.5), X);  // 0.5*(eye+diss)*(...)
+  }
+  if (A.use_urhs) r = r + rmul((T)A.urhs, dpx);
+  if (A.eqn == 3) {
+    T rr = sqrt((x * x + y * y) + z * z);
+    T sph = (T)0.5 * ((T)1.0 + tanh((rr - (T)10.0) - (T)2.0));  // sphere(), ic.f90:640-662
+    r = mk(r.re * sph, r.im * sph);
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+template <int KIND>
+struct KindInfo {
+  static constexpr int nin =
+      (KIND == K_RK2_2 || KIND == K_RK4_2 || KIND == K_RK4_3 || KIND == K_RK4_4) ? 1
+      : (KIND >= K_CK_2 && KIND <= K_CK_6)                                           ? (KIND - K_CK_1)
+                                                                                     : 0;
+  static constexpr bool complex_coef = (KIND == K_RK2_2 || KIND == K_RK4_2 || KIND == K_RK4_3 || KIND == K_RK4_4);
+  static constexpr bool final_ = (KIND == K_EULER || KIND == K_RK2_2 || KIND == K_RK4_4 || KIND == K_CK_6);
+  // FIFO entries (complex T) per point
+  static constexpr int nf = (KIND == K_RK2_2 || KIND == K_RK4_2) ? 1 : (KIND == K_RK4_4 || KIND == K_CK_6) ? 2 : 0;
+};
+
+template <typename T, int R, int NW, int P, int KIND, bool COMP>
+__global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
+  constexpr int TX = 64, TY = NW * P, LX = TX + 2 * R, LY = TY + 2 * R;
+  constexpr int NQ = 2 * R + 1;
+  using KI = KindInfo<KIND>;
+  constexpr int NIN = KI::nin;
+  constexpr int NF = KI::nf > 0 ? KI::nf : 1;
+  constexpr bool IS_DBL = std::is_same<T, double>::value;
+  // FP32 rk45 final: psi_hi rides in F[0], the error partial in F[1] and the
+  // FP64 increment partial in FD.
+  constexpr bool FDBL = (!IS_DBL && KIND == K_CK_6);
+  using V = vec2_t<T>;
+
+  __shared__ V lds[2][LY][LX];
+  __shared__ double red[3][NW];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+
+  // XCD-aware tile order: the hardware deals consecutive workgroups round
+  // robin over the 8 XCDs; give each XCD a contiguous range of tiles so
+  // neighbouring tiles (which share ghost rows) run on one L2.
+  const int nb = A.nblocks;
+  const int b = blockIdx.x;
+  const int xcd = b & 7, loc = b >> 3;
+  const int q8 = nb >> 3, r8 = nb & 7;
+  const int L = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
+  const int tx = L % A.ntx;
+  const int rest = L / A.ntx;
+  const int ty = rest % A.nty;
+  const int tz = rest / A.nty;
+  const int tx0 = tx * TX, ty0 = ty * TY;
+  const int z0 = tz * A.zc;
+  const int z1 = min(z0 + A.zc, A.nz);
+
+  const int nx = A.nx, ny = A.ny, nz = A.nz, per = A.periodic;
+  const size_t plane = (size_t)nx * (size_t)ny;
+
+  const int gi = tx0 + lane;
+  const bool act_x = gi < nx;
+  const int gim = bcmap(min(gi, nx - 1 + R), nx, per);
+  int gj[P];
+  size_t roff[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    gj[p] = ty0 + wave * P + p;
+    const int gjm = bcmap(min(gj[p], ny - 1 + R), ny, per);
+    roff[p] = (size_t)gjm * nx + gim;
+  }
+
+  const cx<T> H = KIND == K_RK4_4 ? mk((T)A.dtr, (T)A.dti) : mk((T)A.hr, (T)A.hi);
+
+  // Stage input u at one site, plus (wantF) the pointwise partials for the
+  // FIFO.  Ghost cells need u only.
+  auto load_u = [&](size_t s, cx<T> &u, cx<T> *F, cx<double> &FD, const bool wantF) {
+    cx<T> ps = ldc<T>(A.psi, s);
+    if constexpr (NIN == 0) {
+      u = ps;
+    } else if constexpr (KI::complex_coef) {
+      cx<T> k0 = ldc<T>(A.kin[0], s);
+      u = ps + cmul(H, k0);
+      if (!wantF) return;
+      if constexpr (KIND == K_RK2_2) {
+        F[0] = ps;
+      } else if constexpr (KIND == K_RK4_2) {
+        F[0] = rdiv(k0, A.d6, 1.0 / 6.0);  // kk1/6
+      } else if constexpr (KIND == K_RK4_4) {
+        cx<T> ac = ldc<T>(A.acc_in, s);
+        F[0] = ps;
+        F[1] = ac + rdiv(k0, A.d3, 1.0 / 3.0);  // (kk1/6 + kk2/3) + kk3/3
+      }
+    } else {
+      cx<T> kk[NIN];
+#pragma unroll
+      for (int m = 0; m < NIN; ++m) kk[m] = ldc<T>(A.kin[m], s);
+      u = ps;
+#pragma unroll
+      for (int m = 0; m < NIN; ++m) u = u + rmul((T)A.b[m], kk[m]);
+      if (!wantF) return;
+      if constexpr (KIND == K_CK_6) {
+        if constexpr (IS_DBL) {
+          cx<T> pn = ps;
+#pragma unroll
+          for (int m = 0; m < 5; ++m) pn = pn + rmul((T)A.c[m + 1], kk[m]);
+          cx<T> pe = rmul((T)A.dc[1], kk[0]);
+#pragma unroll
+          for (int m = 1; m < 5; ++m) pe = pe + rmul((T)A.dc[m + 1], kk[m]);
+          F[0] = pn;
+          F[1] = pe;
+        } else {
+          cx<double> pd = rmul(A.c[1], todbl(kk[0]));
+          pd = pd + rmul(A.c[3], todbl(kk[2]));
+          pd = pd + rmul(A.c[4], todbl(kk[3]));
+          cx<T> pe = rmul((T)A.dc[1], kk[0]);
+#pragma unroll
+          for (int m = 1; m < 5; ++m) pe = pe + rmul((T)A.dc[m + 1], kk[m]);
+          F[0] = ps;
+          F[1] = pe;
+          FD = pd;
+        }
+      }
+    }
+  };
+
+  cx<T> q[NQ][P];
+  cx<T> F[R + 1][P][NF];
+  cx<double> FD[R + 1][P];
+
+  // Prime the queue with planes z0-R .. z0+R-1.
+#pragma unroll
+  for (int m = 0; m < 2 * R; ++m) {
+    const int zp = bcmap(z0 - R + m, nz, per);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      cx<T> Ft[NF];
+      cx<double> FDt;
+      load_u((size_t)zp * plane + roff[p], q[m][p], Ft, FDt, m >= R);
+      if (m >= R) {
+        if constexpr (KI::nf > 0) {
+#pragma unroll
+          for (int f = 0; f < NF; ++f) F[m - R][p][f] = Ft[f];
+        }
+        if constexpr (FDBL) FD[m - R][p] = FDt;
+      }
+    }
+  }
+
+  double acc_norm = 0.0, acc_emax = 0.0, acc_en = 0.0;
+  const int nthreads = 64 * NW;
+  constexpr int NYH = 2 * R * TX, NXH = 2 * R * TY, NH = NYH + NXH;
+
+  for (int k = z0; k < z1; ++k) {
+    const int buf = (k - z0) & 1;
+    // new plane k+R into the queue tail and FIFO tail
+    {
+      const int zp = bcmap(k + R, nz, per);
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        cx<T> Ft[NF];
+        cx<double> FDt;
+        load_u((size_t)zp * plane + roff[p], q[2 * R][p], Ft, FDt, true);
+        if constexpr (KI::nf > 0) {
+#pragma unroll
+          for (int f = 0; f < NF; ++f) F[R][p][f] = Ft[f];
+        }
+        if constexpr (FDBL) FD[R][p] = FDt;
+      }
+    }
+    // ghost cells of plane k
+    for (int h = threadIdx.x; h < NH; h += nthreads) {
+      int lr, lc;
+      if (h < NYH) {
+        const int rs = h / TX, c = h - rs * TX;
+        lr = rs < R ? rs : TY + rs;
+        lc = c + R;
+      } else {
+        const int hh = h - NYH;
+        const int r = hh / (2 * R), cs = hh - r * (2 * R);
+        lr = r + R;
+        lc = cs < R ? cs : TX + cs;
+      }
+      const int hgi = bcmap(min(tx0 + lc - R, nx - 1 + R), nx, per);
+      const int hgj = bcmap(min(ty0 + lr - R, ny - 1 + R), ny, per);
+      cx<T> u, Ft[NF];
+      cx<double> FDt;
+      load_u((size_t)k * plane + (size_t)hgj * nx + hgi, u, Ft, FDt, false);
+      V v;
+      v.x = u.re;
+      v.y = u.im;
+      lds[buf][lr][lc] = v;
+    }
+    // plane k interior from the queue centre
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      V v;
+      v.x = q[R][p].re;
+      v.y = q[R][p].im;
+      lds[buf][wave * P + p + R][lane + R] = v;
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int lr = wave * P + p + R, lc = lane + R;
+      auto L2 = [&](int r, int c) {
+        V v = lds[buf][r][c];
+        return mk<T>(v.x, v.y);
+      };
+      const cx<T> c0 = q[R][p];
+      cx<T> xm1 = L2(lr, lc - 1), xp1 = L2(lr, lc + 1), ym1 = L2(lr - 1, lc), yp1 = L2(lr + 1, lc);
+      cx<T> xm2 = xm1, xp2 = xp1, ym2 = ym1, yp2 = yp1, zm2, zp2;
+      if constexpr (R == 2) {
+        xm2 = L2(lr, lc - 2);
+        xp2 = L2(lr, lc + 2);
+        ym2 = L2(lr - 2, lc);
+        yp2 = L2(lr + 2, lc);
+        zm2 = q[0][p];
+        zp2 = q[4][p];
+      } else {
+        zm2 = q[0][p];
+        zp2 = q[2][p];
+      }
+      const cx<T> zm1 = q[R - 1][p], zp1 = q[R + 1][p];
+      cx<T> dxx = d2<T, R>(xm2, xm1, c0, xp1, xp2, A.d_xx, A.inv_xx);
+      cx<T> dyy = d2<T, R>(ym2, ym1, c0, yp1, yp2, A.d_yy, A.inv_yy);
+      cx<T> dzz = d2<T, R>(zm2, zm1, c0, zp1, zp2, A.d_zz, A.inv_zz);
+      cx<T> lap = (dxx + dyy) + dzz;
+      const bool act = act_x && (gj[p] < ny);
+      const size_t s = (size_t)k * plane + (size_t)gj[p] * nx + gi;
+
+      if constexpr (KIND == K_ENERGY) {
+        if (act) {
+          cx<T> pr = cmul(mk(-lap.re, -lap.im), mk(c0.re, -c0.im));
+          T m = cabs_(c0);
+          T m2 = m * m;
+          acc_en += (double)(pr.re + (T)0.5 * (m2 * m2));
+        }
+      } else {
+        cx<T> dpx = mk((T)0, (T)0);
+        if (A.use_urhs) dpx = d1<T, R>(xm2, xm1, xp1, xp2, A.d_x, A.inv_x);
+        const cx<T> f = rhs_site<T>(A, c0, lap, dpx, gi, gj[p], k);
+        if (act) {
+          if constexpr (KIND == K_RHS || KIND == K_RK2_1 || KIND == K_RK4_1 || KIND == K_RK4_3) {
+            stc<T>(A.kout, s, f);
+          } else if constexpr (KIND == K_RK4_2) {
+            stc<T>(A.kout, s, f);
+            stc<T>(A.acc_out, s, F[0][p][0] + rdiv(f, A.d3, 1.0 / 3.0));
+          } else if constexpr (KIND >= K_CK_1 && KIND <= K_CK_6) {
+            cx<T> kk;
+            if constexpr (IS_DBL) {
+              kk = cmul(mk((T)A.dtr, (T)A.dti), f);
+            } else {
+              cx<double> kd = cmul(mk(A.dtr, A.dti), todbl(f));
+              kk = mk((T)kd.re, (T)kd.im);
+            }
+            if constexpr (KIND == K_CK_1) {
+              stc<T>(A.kout, s, kk);
+              if (A.write_scale) {
+                // psi_scal = abs(in) + dt*abs(f) + 1e-30 (solve.f90:176-178)
+                T af = cabs_(f), au = cabs_(c0);
+                cx<T> sc = mk((au + (T)A.sdtr * af) + (T)1e-30, (T)A.sdti * af);
+                stc<T>(A.scale_out, s, sc);
+              }
+            } else if constexpr (KIND < K_CK_6) {
+              stc<T>(A.kout, s, kk);
+            } else {
+              // K_CK_6: new state, error norm
+              const cx<T> sc = ldc<T>(A.scale_in, s);
+              cx<T> err = F[0][p][1] + rmul((T)A.dc[6], kk);
+              cx<double> np;
+              if constexpr (IS_DBL) {
+                np = F[0][p][0] + rmul((T)A.c[6], kk);
+                stc<T>(A.psi_out, s, np);
+                cx<double> q2 = cdiv(err, sc);
+                double e = cabs_(q2);
+                acc_emax = (e > acc_emax || e != e) ? e : acc_emax;
+              } else {
+                cx<double> dl = FD[0][p] + rmul(A.c[6], todbl(kk));
+                cx<double> base = todbl(F[0][p][0]);
+                if constexpr (COMP) base = base + todbl(ldc<T>(A.lo, s));
+                np = base + dl;
+                cx<T> hi = mk((T)np.re, (T)np.im);
+                stc<T>(A.psi_out, s, hi);
+                if constexpr (COMP) stc<T>(A.lo_out, s, mk((T)(np.re - (double)hi.re), (T)(np.im - (double)hi.im)));
+                float e = sqrtf(err.re * err.re + err.im * err.im) / sqrtf(sc.re * sc.re + sc.im * sc.im);
+                acc_emax = ((double)e > acc_emax || e != e) ? (double)e : acc_emax;
+              }
+              const double wn = nweight(gi, nx, A.w1, A.w2, A.w3) * nweight(gj[p], ny, A.w1, A.w2, A.w3) *
+                                nweight(k, nz, A.w1, A.w2, A.w3);
+              acc_norm += wn * (np.re * np.re + np.im * np.im);
+            }
+          } else if constexpr (KI::final_) {
+            // K_EULER, K_RK2_2, K_RK4_4
+            cx<T> ps;
+            cx<T> incr_t;  // the complex the final dt multiplies
+            if constexpr (KIND == K_EULER) {
+              ps = c0;
+              incr_t = f;
+            } else if constexpr (KIND == K_RK2_2) {
+              ps = F[0][p][0];
+              incr_t = f;
+            } else {
+              ps = F[0][p][0];
+              incr_t = F[0][p][1] + rdiv(f, A.d6, 1.0 / 6.0);
+            }
+            cx<double> np;
+            if constexpr (IS_DBL) {
+              np = ps + cmul(mk((T)A.dtr, (T)A.dti), incr_t);
+              stc<T>(A.psi_out, s, np);
+            } else {
+              cx<double> base = todbl(ps);
+              if constexpr (COMP) base = base + todbl(ldc<T>(A.lo, s));
+              np = base + cmul(mk(A.dtr, A.dti), todbl(incr_t));
+              cx<T> hi = mk((T)np.re, (T)np.im);
+              stc<T>(A.psi_out, s, hi);
+              if constexpr (COMP) stc<T>(A.lo_out, s, mk((T)(np.re - (double)hi.re), (T)(np.im - (double)hi.im)));
+            }
+            const double wn = nweight(gi, nx, A.w1, A.w2, A.w3) * nweight(gj[p], ny, A.w1, A.w2, A.w3) *
+                              nweight(k, nz, A.w1, A.w2, A.w3);
+            acc_norm += wn * (np.re * np.re + np.im * np.im);
+          }
+        }
+      }
+    }
+
+    // shift queue and FIFO
+#pragma unroll
+    for (int m = 0; m < 2 * R; ++m)
+#pragma unroll
+      for (int p = 0; p < P; ++p) q[m][p] = q[m + 1][p];
+#pragma unroll
+    for (int m = 0; m < R; ++m)
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        if constexpr (KI::nf > 0) {
+#pragma unroll
+          for (int f = 0; f < NF; ++f) F[m][p][f] = F[m + 1][p][f];
+        }
+        if constexpr (FDBL) FD[m][p] = FD[m + 1][p];
+      }
+  }
+
+  // Deterministic block reductions: xor-shuffle within the wave, then the
+  // waves in order.
+  if constexpr (KI::final_ || KIND == K_ENERGY) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      acc_norm += __shfl_xor(acc_norm, o);
+      acc_en += __shfl_xor(acc_en, o);
+      double om = __shfl_xor(acc_emax, o);
+      acc_emax = (om > acc_emax || om != om) ? om : acc_emax;
+    }
+    if (lane == 0) {
+      red[0][wave] = acc_norm;
+      red[1][wave] = acc_emax;
+      red[2][wave] = acc_en;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sn = 0.0, se = 0.0, mx = 0.0;
+      for (int w = 0; w < NW; ++w) {
+        sn += red[0][w];
+        se += red[2][w];
+        mx = (red[1][w] > mx || red[1][w] != red[1][w]) ? red[1][w] : mx;
+      }
+      if (A.part_norm) A.part_norm[b] = sn;
+      if (A.part_emax) A.part_emax[b] = mx;
+      if (A.part_energy) A.part_energy[b] = se;
+    }
+  }
+}
+
+// Sum / max of the per-block partials in a fixed order (one workgroup).
+// out[0] = sum(part_norm) * scale_norm, out[1] = max(part_emax) / eps,
+// out[2] = sum(part_energy) * scale_energy; hist (if set) gets out[0].
+static __global__ void __launch_bounds__(256) reduce_partials(const double *pn, const double *pe, const double *pen, int n,
+                                                       double scale_norm, double inv_eps, double scale_energy,
+                                                       double *out, double *hist) {
+  __shared__ double s0[256], s1[256], s2[256];
+  double a = 0.0, m = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    if (pn) a += pn[i];
+    if (pe) m = (pe[i] > m || pe[i] != pe[i]) ? pe[i] : m;
+    if (pen) c += pen[i];
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = m;
+  s2[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      double o = s1[threadIdx.x + w];
+      s1[threadIdx.x] = (o > s1[threadIdx.x] || o != o) ? o : s1[threadIdx.x];
+      s2[threadIdx.x] += s2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s0[0] * scale_norm;
+    out[1] = s1[0] * inv_eps;
+    out[2] = s2[0] * scale_energy;
+    if (hist) *hist = out[0];
+  }
+}
+
+}  // namespace gpe

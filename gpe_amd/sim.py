@@ -1,0 +1,171 @@
+"""Python host interface of the GP time-stepping path.
+
+`Simulation` owns one device context (one field on the GPU) and exposes the
+reference's operations on it: solver() calls (gpe_sim_advance / the drop-in
+solver_step), get_norm, energy, mass, renormalise and get_rhs.  Fields are
+numpy arrays of shape (nz, ny, nx) -- x fastest, the Fortran order of the
+reference's psi(0:nx1, js:je, ks:ke).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .config import RunConfig
+
+
+def make_params(cfg: RunConfig) -> _lib.GpeParams:
+    p = _lib.GpeParams()
+    p.nx, p.ny, p.nz = cfg.nx, cfg.ny, cfg.nz
+    p.scheme = _lib.SCHEME_CODES[cfg.scheme]
+    p.eqn, p.bcs, p.order = cfg.eqn, cfg.bcs, cfg.order
+    p.precision = _lib.PRECISION_CODES[cfg.precision]
+    p.real_time = 1 if cfg.real_time else 0
+    p.xl, p.yl, p.zl = cfg.xl, cfg.yl, cfg.zl
+    p.dx, p.dy, p.dz = cfg.dx, cfg.dy, cfg.dz
+    p.g, p.mu = cfg.g, cfg.mu
+    p.omx, p.omy, p.omz = cfg.omx, cfg.omy, cfg.omz
+    p.urhs, p.diss_amp = cfg.urhs, cfg.diss_amp
+    p.eps, p.safety = cfg.eps, cfg.safety
+    p.dt_increase, p.dt_decrease = cfg.dt_increase, cfg.dt_decrease
+    return p
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Simulation:
+    """One GP simulation resident on the GPU."""
+
+    def __init__(self, cfg: RunConfig, psi=None, dt=None, t=0.0, im_t=0.0, p=0):
+        self.cfg = cfg
+        self._L = _lib.lib()
+        self._ctx = ctypes.c_void_p()
+        _lib.check(self._L.gpe_ctx_create(ctypes.byref(make_params(cfg)), ctypes.byref(self._ctx)))
+        self.set_state(cfg.dt if dt is None else dt, t=t, im_t=im_t, p=p)
+        if psi is not None:
+            self.upload(psi)
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if self._ctx:
+            self._L.gpe_ctx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state --------------------------------------------------------------
+    @property
+    def host_dtype(self):
+        return np.complex64 if self.cfg.precision == "fp32" else np.complex128
+
+    def set_state(self, dt, t=0.0, im_t=0.0, p=0):
+        s = _lib.GpeStepState()
+        dt = complex(dt)
+        s.dt_re, s.dt_im, s.t, s.im_t, s.p = dt.real, dt.imag, t, im_t, p
+        _lib.check(self._L.gpe_sim_set_state(self._ctx, ctypes.byref(s)))
+
+    def state(self):
+        s = _lib.GpeStepState()
+        _lib.check(self._L.gpe_sim_get_state(self._ctx, ctypes.byref(s)))
+        return {"dt": complex(s.dt_re, s.dt_im), "t": s.t, "im_t": s.im_t, "errmax": s.errmax,
+                "n_rejects": s.n_rejects, "p": s.p}
+
+    # -- data ---------------------------------------------------------------
+    def upload(self, psi):
+        psi = np.ascontiguousarray(psi)
+        if psi.shape != self.cfg.shape:
+            raise ValueError("field shape %s != %s" % (psi.shape, self.cfg.shape))
+        if psi.dtype == np.complex128:
+            ht = _lib.GPE_HOST_C128
+        elif psi.dtype == np.complex64:
+            ht = _lib.GPE_HOST_C64
+        else:
+            raise TypeError("complex128 or complex64 field expected")
+        _lib.check(self._L.gpe_sim_upload(self._ctx, _ptr(psi), ht))
+
+    def download(self, dtype=None):
+        dtype = np.dtype(dtype or (np.complex64 if self.cfg.precision == "fp32" else np.complex128))
+        out = np.empty(self.cfg.shape, dtype=dtype)
+        ht = _lib.GPE_HOST_C128 if dtype == np.complex128 else _lib.GPE_HOST_C64
+        _lib.check(self._L.gpe_sim_download(self._ctx, _ptr(out), ht))
+        return out
+
+    # -- stepping -----------------------------------------------------------
+    def advance(self, nsteps, record_norm=False):
+        """nsteps solver() calls; returns the per-step get_norm if asked."""
+        _lib.check(self._L.gpe_sim_advance(self._ctx, int(nsteps), 1 if record_norm else 0))
+        if record_norm:
+            h = np.empty(int(nsteps), dtype=np.float64)
+            _lib.check(self._L.gpe_sim_norm_history(
+                self._ctx, h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(nsteps)))
+            return h
+        return None
+
+    def solver_step(self, var_in, halo=2):
+        """The drop-in call: var_in has `halo` ghost layers in y and z, shape
+        (nz+2h, ny+2h, nx); returns var_out (nz, ny, nx) and updates the
+        state as solver() does."""
+        var_in = np.ascontiguousarray(var_in)
+        nz, ny, nx = self.cfg.shape
+        if var_in.shape != (nz + 2 * halo, ny + 2 * halo, nx):
+            raise ValueError("var_in shape %s" % (var_in.shape,))
+        out = np.empty(self.cfg.shape, dtype=var_in.dtype)
+        s = _lib.GpeStepState()
+        _lib.check(self._L.gpe_sim_get_state(self._ctx, ctypes.byref(s)))
+        _lib.check(self._L.gpe_solver_step(self._ctx, _ptr(var_in), int(halo), _ptr(out), ctypes.byref(s)))
+        _lib.check(self._L.gpe_sim_set_state(self._ctx, ctypes.byref(s)))
+        return out
+
+    # -- reductions ---------------------------------------------------------
+    def _scalar(self, fn):
+        v = ctypes.c_double()
+        _lib.check(fn(self._ctx, ctypes.byref(v)))
+        return v.value
+
+    def norm(self):
+        return self._scalar(self._L.gpe_sim_norm)
+
+    def energy(self):
+        return self._scalar(self._L.gpe_sim_energy)
+
+    def mass(self):
+        return self._scalar(self._L.gpe_sim_mass)
+
+    def renormalise(self, nn, norm):
+        _lib.check(self._L.gpe_sim_renormalise(self._ctx, float(nn), float(norm)))
+
+    def rhs(self):
+        dtype = np.complex64 if self.cfg.precision == "fp32" else np.complex128
+        out = np.empty(self.cfg.shape, dtype=dtype)
+        ht = _lib.GPE_HOST_C128 if dtype == np.complex128 else _lib.GPE_HOST_C64
+        _lib.check(self._L.gpe_sim_rhs(self._ctx, _ptr(out), ht))
+        return out
+
+    # -- kernel timing ------------------------------------------------------
+    def prof_enable(self, on=True):
+        _lib.check(self._L.gpe_prof_enable(self._ctx, 1 if on else 0))
+
+    def prof_query(self):
+        """{kind name: (launches, total_ms, algorithmic bytes per launch)}"""
+        out = {}
+        for k, nm in enumerate(_lib.KIND_NAMES):
+            n = ctypes.c_int64()
+            ms = ctypes.c_double()
+            by = ctypes.c_double()
+            name = ctypes.create_string_buffer(64)
+            _lib.check(self._L.gpe_prof_query(self._ctx, k, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by), name))
+            if n.value:
+                out[nm] = (n.value, ms.value, by.value)
+        return out

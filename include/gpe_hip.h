@@ -1,0 +1,137 @@
+/*
+ * gpe_hip.h -- C ABI of the MI355X (gfx950) GP time-stepping library.
+ *
+ * The library replaces the reference's time-stepping hot path: the GP
+ * right-hand side (solve.f90:382-440) on the 2nd/4th-order centred-FD
+ * Laplacian (derivs.f90:146-272, variables.f90:55-71), fused into the
+ * Euler / RK2 / RK4 / Cash-Karp RKF45 stage updates (solve.f90:57-378),
+ * with the norm (variables.f90:554-576, 730-840), energy
+ * (variables.f90:580-604) and adaptive-error (solve.f90:176-190)
+ * reductions on the device.
+ *
+ * Plain C: pointers, sizes and POD structs only.  No call aborts; every
+ * entry point returns GPE_OK (0) or a negative GPE_E* code, and
+ * gpe_last_error() gives the message of the last failure on the calling
+ * thread.  One host thread per context.
+ *
+ * Field layout (host and device): nx*ny*nz complex values, real and
+ * imaginary parts interleaved, x fastest -- the Fortran order of
+ * psi(0:nx1, js:je, ks:ke) on one rank.
+ */
+#ifndef GPE_HIP_H
+#define GPE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes. */
+#define GPE_OK 0
+#define GPE_EINVAL (-1)   /* bad argument or parameter */
+#define GPE_ENOMEM (-2)   /* device or host allocation failed */
+#define GPE_EHIP (-3)     /* a HIP runtime call failed */
+#define GPE_ENONFINITE (-4) /* NaN in the adaptive error norm */
+#define GPE_ESTEP (-5)    /* adaptive step control gave up (too many rejections) */
+#define GPE_ESTATE (-6)   /* call out of order (e.g. no field uploaded) */
+
+/* scheme (run_params%scheme, solve.f90:34-50) */
+#define GPE_EULER 1
+#define GPE_RK2 2
+#define GPE_RK4 4
+#define GPE_RK45 45
+
+/* precision of the device path */
+#define GPE_FP64 0  /* complex128 storage and arithmetic: the parity path */
+#define GPE_FP32 1  /* complex64 storage and arithmetic */
+#define GPE_FP32C 2 /* complex64 storage + hi/lo compensated psi update,
+                       FP64 reductions and step control */
+
+/* host buffer element type for uploads and downloads */
+#define GPE_HOST_C128 0
+#define GPE_HOST_C64 1
+
+/* Run parameters: run_params / eqn_params (parameters.f90:24-56), the
+ * grid from get_grid (ic.f90:32-69) and the adaptive-step constants
+ * (parameters.f90:90-93). */
+typedef struct gpe_params {
+  int32_t nx, ny, nz;      /* grid (parameters.in:27-29); each >= 5 */
+  int32_t scheme;          /* GPE_EULER | GPE_RK2 | GPE_RK4 | GPE_RK45 */
+  int32_t eqn;             /* eqn_to_solve 1..4 */
+  int32_t bcs;             /* 1 periodic, 2 reflective */
+  int32_t order;           /* 2 or 4 */
+  int32_t precision;       /* GPE_FP64 | GPE_FP32 | GPE_FP32C */
+  int32_t real_time;       /* 1 real time, 0 imaginary time */
+  int32_t reserved;
+  double xl, yl, zl;       /* -xr, -yr, -zr */
+  double dx, dy, dz;       /* (xr-xl)/(nx-1), ... */
+  double g, mu;            /* eqn 4 coefficients */
+  double omx, omy, omz;    /* trap frequencies (Vtrap, ic.f90:713-731) */
+  double urhs, diss_amp;   /* eqns 1-3 */
+  double eps, safety, dt_increase, dt_decrease; /* rk45 control */
+} gpe_params;
+
+/* Time-stepping state carried across calls (parameters.f90:117-119,129). */
+typedef struct gpe_step_state {
+  double dt_re, dt_im;     /* complex dt; rk45 updates it */
+  double t, im_t;
+  double errmax;           /* out: errmax of the last accepted rk45 attempt */
+  int32_t n_rejects;       /* out: rejected rk45 attempts in the last call */
+  int32_t p;               /* step counter, advanced by gpe_sim_advance */
+} gpe_step_state;
+
+typedef struct gpe_ctx gpe_ctx;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *gpe_last_error(void);
+
+/* Create a context for one simulation: validates params, allocates the
+ * device fields the scheme needs and a stream.  Replaces the automatic
+ * arrays of rk4/rkck (solve.f90:117-121, 284-290). */
+int gpe_ctx_create(const gpe_params *params, gpe_ctx **out);
+void gpe_ctx_destroy(gpe_ctx *ctx);
+
+/* Drop-in for solver(var_in, var_out) (solve.f90:25-53), as called by
+ * gpe.f90:230.  var_in has `halo` ghost layers in y and z (2 for the
+ * Fortran caller: psi%old(0:nx1, js-2:je+2, ks-2:ke+2)); only its interior
+ * is read -- boundaries are applied in the kernels.  var_out is
+ * (nx, ny, nz).  Element type: complex128 for GPE_FP64, complex64 for
+ * GPE_FP32.  st is updated as solver() updates t, im_t and dt.  Synchronous
+ * with respect to both host buffers. */
+int gpe_solver_step(gpe_ctx *ctx, const void *var_in, int32_t halo, void *var_out,
+                    gpe_step_state *st);
+
+/* Device-resident path (no host traffic per step). */
+int gpe_sim_upload(gpe_ctx *ctx, const void *psi, int32_t host_type);
+int gpe_sim_download(gpe_ctx *ctx, void *psi, int32_t host_type);
+int gpe_sim_set_state(gpe_ctx *ctx, const gpe_step_state *st);
+int gpe_sim_get_state(gpe_ctx *ctx, gpe_step_state *st);
+/* nsteps solver() calls on the resident field (the time loop of
+ * gpe.f90:136-280 without its host I/O).  If record_norm is nonzero,
+ * get_norm of each new state is kept (gpe.f90:243-245); read it with
+ * gpe_sim_norm_history. */
+int gpe_sim_advance(gpe_ctx *ctx, int64_t nsteps, int32_t record_norm);
+int gpe_sim_norm_history(gpe_ctx *ctx, double *out, int64_t n);
+/* Reductions of the resident field, in the reference's functional forms. */
+int gpe_sim_norm(gpe_ctx *ctx, double *norm);     /* get_norm */
+int gpe_sim_energy(gpe_ctx *ctx, double *energy); /* energy */
+int gpe_sim_mass(gpe_ctx *ctx, double *mass);     /* mass */
+/* renormalise (variables.f90:715-726): psi = sqrt(nn)*psi/sqrt(norm). */
+int gpe_sim_renormalise(gpe_ctx *ctx, double nn, double norm);
+/* f(psi) of the resident field into a host buffer (get_rhs, solve.f90:382). */
+int gpe_sim_rhs(gpe_ctx *ctx, void *out, int32_t host_type);
+
+/* Kernel timing with HIP events recorded on the context's stream around
+ * every stage launch.  enable=1 resets the counters. */
+int gpe_prof_enable(gpe_ctx *ctx, int32_t enable);
+/* For kernel kind `kind` (0..GPE_PROF_KINDS-1): launches, total ms and
+ * algorithmic HBM bytes per launch; the name is written to name[64]. */
+#define GPE_PROF_KINDS 16
+int gpe_prof_query(gpe_ctx *ctx, int32_t kind, int64_t *launches, double *total_ms,
+                   double *bytes_per_launch, char *name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPE_HIP_H */

@@ -1,0 +1,56 @@
+"""The C-ABI library loads and exports every entry point include/gpe_hip.h
+declares (no GPU needed: nothing here launches work)."""
+import ctypes
+import os
+import re
+
+from gpe_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(gpe_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    assert {"gpe_ctx_create", "gpe_solver_step", "gpe_sim_advance", "gpe_last_error"} <= names
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in sorted(declared_symbols()) if not hasattr(L, n)]
+    assert not missing, "not exported: %s" % missing
+
+
+def test_python_binding_covers_header():
+    assert declared_symbols() == set(_lib.EXPORTS)
+
+
+def test_struct_layouts_match_header():
+    # gpe_params: 10 int32 then 17 doubles; gpe_step_state: 5 doubles, 2 int32
+    assert ctypes.sizeof(_lib.GpeParams) == 10 * 4 + 17 * 8
+    assert ctypes.sizeof(_lib.GpeStepState) == 5 * 8 + 2 * 4
+
+
+def test_invalid_params_rejected_without_gpu_work():
+    # parameter validation happens before any device call
+    from gpe_amd.sim import make_params
+    from gpe_amd.config import RunConfig
+    L = _lib.lib()
+    p = make_params(RunConfig(nx=4, ny=8, nz=8))
+    ctx = ctypes.c_void_p()
+    rc = L.gpe_ctx_create(ctypes.byref(p), ctypes.byref(ctx))
+    assert rc == -1 and not ctx
+    assert b"at least 5" in L.gpe_last_error()
+    p = make_params(RunConfig(nx=8, ny=8, nz=8))
+    p.scheme = 3
+    assert L.gpe_ctx_create(ctypes.byref(p), ctypes.byref(ctx)) == -1
+    assert b"Unrecognised time stepping scheme" in L.gpe_last_error()

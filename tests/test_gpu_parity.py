@@ -1,0 +1,263 @@
+"""GPU parity: the HIP library against the oracle on the same seeded inputs.
+
+FP64 is the drop-in parity path: <= 1e-12 relative L2 in psi, norm and
+energy (the bar the reference's FP64 path is held to).  The FP32 forms are
+held to 1e-5 relative L2 against the FP64 oracle.  Every test prints the
+case before touching the GPU.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from gpe_amd import ic
+from gpe_amd.config import RunConfig, xr_for_spacing
+
+pytestmark = pytest.mark.gpu
+
+NT = min(16, os.cpu_count() or 1)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def small_cfg(**kw):
+    # non-cubic, x spans two 64-wide tiles (the second partial), odd y and z
+    base = dict(nx=70, ny=21, nz=19, xr=7.0, yr=4.0, zr=3.6, eqn=4, g=1.0, mu=1.0,
+                omx=0.3, omy=0.2, omz=0.4, tau=0.005)
+    base.update(kw)
+    return RunConfig(**base)
+
+
+def ring_ic(cfg):
+    psi = ic.vortex_ring(cfg, 2.0)
+    rng = np.random.default_rng(7)
+    return psi * (1 + 0.05 * rng.standard_normal(cfg.shape))
+
+
+@pytest.mark.parametrize("eqn", [1, 2, 3, 4])
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("bcs", [1, 2])
+def test_rhs_fp64(gpu, oracle, eqn, order, bcs):
+    from gpe_amd.sim import Simulation
+    extra = {}
+    if eqn in (1, 3):
+        extra = dict(diss_amp=0.3, urhs=0.2)
+    elif eqn == 2:
+        extra = dict(urhs=-0.1)
+    cfg = small_cfg(eqn=eqn, order=order, bcs=bcs, **extra)
+    psi = ring_ic(cfg)
+    say("rhs fp64", eqn, order, bcs, cfg.shape)
+    with Simulation(cfg, psi) as sim:
+        got = sim.rhs()
+    want = oracle.rhs(cfg, psi, NT)
+    assert rel_l2(got, want) < 1e-14
+    if eqn == 4:
+        # eqn 4 uses no transcendental except hypot: nearly every site is bitwise equal
+        assert np.mean(got == want) > 0.95
+
+
+@pytest.mark.parametrize("scheme", ["euler", "rk2", "rk4", "rk45"])
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("bcs", [1, 2])
+def test_steps_fp64(gpu, oracle, scheme, order, bcs):
+    from gpe_amd.sim import Simulation
+    cfg = small_cfg(scheme=scheme, order=order, bcs=bcs, tau=0.002 if scheme == "euler" else 0.005)
+    psi = ring_ic(cfg)
+    n = 30
+    say("steps fp64", scheme, order, bcs, n)
+    st = oracle.make_state(cfg.dt)
+    want, wnorms = oracle.run(cfg, st, psi, n, NT, with_norms=True)
+    with Simulation(cfg, psi) as sim:
+        gnorms = sim.advance(n, record_norm=True)
+        got = sim.download()
+        gst = sim.state()
+    assert rel_l2(got, want) < 1e-12
+    assert np.max(np.abs(gnorms - wnorms) / wnorms) < 1e-12
+    assert gst["t"] == pytest.approx(st.t, rel=1e-14, abs=1e-15)
+    assert gst["dt"].real == pytest.approx(st.dt_re, rel=1e-12)
+    assert gst["p"] == n
+
+
+def test_rk45_step_control_fp64(gpu, oracle):
+    # a too-large first step: rejections and dt adaptation must match
+    from gpe_amd.sim import Simulation
+    cfg = small_cfg(scheme="rk45", order=4, bcs=2, tau=0.5)
+    psi = ring_ic(cfg)
+    say("rk45 control", cfg.shape)
+    st = oracle.make_state(cfg.dt)
+    rejects, dts = [], []
+    cur = psi
+    for _ in range(6):
+        cur = oracle.solver(cfg, st, cur, NT)
+        rejects.append(st.n_rejects)
+        dts.append(st.dt_re)
+    with Simulation(cfg, psi) as sim:
+        g_rej, g_dts = [], []
+        for _ in range(6):
+            sim.advance(1)
+            s = sim.state()
+            g_rej.append(s["n_rejects"])
+            g_dts.append(s["dt"].real)
+        got = sim.download()
+    assert sum(rejects) > 0
+    assert g_rej == rejects
+    assert np.allclose(g_dts, dts, rtol=1e-12, atol=0)
+    assert rel_l2(got, cur) < 1e-12
+
+
+def test_imaginary_time_rk2_renormalise(gpu, oracle):
+    # trap_vortex-like: imaginary time, rk2, renormalisation by the norm
+    from gpe_amd.sim import Simulation
+    cfg = small_cfg(scheme="rk2", order=2, bcs=2, real_time=False, tau=0.001, g=5.0, mu=8.0)
+    psi = ring_ic(cfg)
+    say("imaginary time rk2")
+    st = oracle.make_state(cfg.dt)
+    cur = psi
+    with Simulation(cfg, psi) as sim:
+        for _ in range(5):
+            cur = oracle.solver(cfg, st, cur, NT)
+            nrm = oracle.norm(cfg, cur)
+            cur = oracle.renormalise(cfg, cur, 1.0, nrm)
+            sim.advance(1)
+            gn = sim.norm()
+            assert gn == pytest.approx(nrm, rel=1e-12)
+            sim.renormalise(1.0, gn)
+        got = sim.download()
+        s = sim.state()
+    assert rel_l2(got, cur) < 1e-12
+    assert s["im_t"] == pytest.approx(st.im_t, rel=1e-14)
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_reductions_fp64(gpu, oracle, order):
+    from gpe_amd.sim import Simulation
+    cfg = small_cfg(order=order, bcs=2)
+    psi = ring_ic(cfg)
+    say("reductions", order)
+    with Simulation(cfg, psi) as sim:
+        assert sim.norm() == pytest.approx(oracle.norm(cfg, psi), rel=1e-13)
+        assert sim.energy() == pytest.approx(oracle.energy(cfg, psi), rel=1e-12)
+        assert sim.mass() == pytest.approx(oracle.mass(cfg, psi), rel=1e-13)
+
+
+def test_dropin_solver_step_with_halo(gpu, oracle):
+    # the Fortran-facing call: psi%old(0:nx1, js-2:je+2, ks-2:ke+2) in, interior out
+    from gpe_amd.sim import Simulation
+    cfg = small_cfg(scheme="rk4", order=4, bcs=1)
+    psi = ring_ic(cfg)
+    say("drop-in solver_step")
+    h = 2
+    old = np.full((cfg.nz + 2 * h, cfg.ny + 2 * h, cfg.nx), np.nan + 0j)
+    st = oracle.make_state(cfg.dt)
+    cur = psi
+    with Simulation(cfg) as sim:
+        for _ in range(3):
+            old[h:-h, h:-h, :] = cur
+            out = sim.solver_step(old, halo=h)
+            cur = oracle.solver(cfg, st, cur, NT)
+            assert rel_l2(out, cur) < 1e-13
+        assert sim.state()["t"] == pytest.approx(st.t, rel=1e-15)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+@pytest.mark.parametrize("scheme", ["rk4", "rk45"])
+def test_fp32_forms_track_fp64(gpu, oracle, precision, scheme):
+    from gpe_amd.sim import Simulation
+    cfg64 = small_cfg(scheme=scheme, order=4, bcs=1)
+    cfg = cfg64.with_(precision=precision)
+    psi = ring_ic(cfg64)
+    n = 40
+    say("fp32 form", precision, scheme)
+    want, _ = oracle.run(cfg64, oracle.make_state(cfg64.dt), psi, n, NT)
+    with Simulation(cfg, psi) as sim:
+        sim.advance(n)
+        got = sim.download(np.complex128)
+    assert rel_l2(got, want) < 1e-5
+
+
+def test_config0_ring_rk4_1000_steps(gpu, oracle):
+    # 64^3, dx = 0.4, reflective, 2nd order, RK4 dt = 0.01, 1000 steps, V = 0
+    cfg = RunConfig(nx=64, ny=64, nz=64, xr=xr_for_spacing(0.4, 64), bcs=2, order=2, scheme="rk4",
+                    tau=0.01, g=1.0, mu=1.0)
+    psi = ic.vortex_ring(cfg, 4.0)
+    say("config 0: 64^3 ring rk4 x1000")
+    from gpe_amd.sim import Simulation
+    st = oracle.make_state(cfg.dt)
+    want, wnorms = oracle.run(cfg, st, psi, 1000, NT, with_norms=True)
+    with Simulation(cfg, psi) as sim:
+        gn = sim.advance(1000, record_norm=True)
+        got = sim.download()
+        ge = sim.energy()
+    err = rel_l2(got, want)
+    say("config 0 rel L2 = %.3e" % err)
+    assert err < 1e-12
+    assert abs(gn[-1] - wnorms[-1]) / wnorms[-1] < 1e-12
+    assert ge == pytest.approx(oracle.energy(cfg, want, NT), rel=1e-12)
+
+
+def test_config1_line_pair_256(gpu, oracle):
+    # 256^3, dx = 0.25, 4th order, reflective, trap, RK4 dt = 0.005 (a few steps)
+    cfg = RunConfig(nx=256, ny=256, nz=256, xr=xr_for_spacing(0.25, 256), bcs=2, order=4, scheme="rk4",
+                    tau=0.005, g=1.0, mu=1.0, omx=0.1, omy=0.1, omz=float(np.sqrt(0.02)))
+    psi = ic.line_pair(cfg)
+    n = 4
+    say("config 1: 256^3 line pair rk4 x%d" % n)
+    from gpe_amd.sim import Simulation
+    want, _ = oracle.run(cfg, oracle.make_state(cfg.dt), psi, n, NT)
+    with Simulation(cfg, psi) as sim:
+        sim.advance(n)
+        got = sim.download()
+    assert rel_l2(got, want) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_plane_wave_512_rk45_matches_scalar_recursion(gpu, precision):
+    # full config-2 size: a plane wave stays a plane wave; every site must
+    # follow the scalar Cash-Karp recursion of its amplitude
+    from gpe_amd.sim import Simulation
+    from tests.test_oracle import _ck_scalar
+    cfg = RunConfig(nx=512, ny=512, nz=512, xr=xr_for_spacing(0.5, 512), bcs=1, order=4, scheme="rk45",
+                    tau=0.05, g=1.0, mu=1.0, precision=precision)
+    m = (3, -2, 5)
+    lam = 0.0
+    for ma, nn, d in ((m[0], cfg.nx, cfg.dx), (m[1], cfg.ny, cfg.dy), (m[2], cfg.nz, cfg.dz)):
+        th = 2 * np.pi * ma / nn
+        lam += (-2 * np.cos(2 * th) + 32 * np.cos(th) - 30) / (12 * d ** 2)
+    a0 = 0.9 + 0.1j
+    f = lambda a: -1j * (-0.5 * lam + abs(a) ** 2 - 1.0) * a  # noqa: E731
+    psi = a0 * ic.plane_wave(cfg, m)
+    say("512^3 plane wave rk45", precision)
+    with Simulation(cfg, psi) as sim:
+        sim.advance(3)
+        s = sim.state()
+        got = sim.download(np.complex128)
+    # scalar control loop from the same dt
+    a, dt, t = a0, 0.05, 0.0
+    for _ in range(3):
+        scale = abs(a) + dt * abs(f(a)) + 1e-30
+        while True:
+            y5, e = _ck_scalar(f, a, dt)
+            em = abs(e) / scale / 1e-6
+            if em <= 1:
+                break
+            dt = max(abs(0.9 * dt * em ** -0.25), 0.1 * abs(dt))
+        t += dt
+        dt = 0.9 * dt * em ** -0.2 if em > (5 / 0.9) ** (1 / -0.2) else 5 * dt
+        a = y5
+    if precision == "fp64":
+        assert s["t"] == pytest.approx(t, rel=1e-9)
+        assert rel_l2(got, a * ic.plane_wave(cfg, m)) < 1e-12
+    else:
+        # still one plane wave: a single amplitude at every site, |a| conserved
+        r = got / ic.plane_wave(cfg, m)
+        rm = r.mean()
+        assert np.max(np.abs(r - rm)) < 2e-6
+        assert abs(abs(rm) - abs(a0)) < 1e-5
