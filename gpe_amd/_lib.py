@@ -28,6 +28,7 @@ EXPORTS = [
     "gpe_sim_upload", "gpe_sim_download", "gpe_sim_set_state", "gpe_sim_get_state",
     "gpe_sim_advance", "gpe_sim_norm_history", "gpe_sim_norm", "gpe_sim_energy",
     "gpe_sim_mass", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
+    "gpe_tune_set", "gpe_tune_get", "gpe_autotune",
 ]
 
 
@@ -93,6 +94,9 @@ def lib():
         "gpe_sim_rhs": ([vp, vp, i32], ctypes.c_int),
         "gpe_prof_enable": ([vp, i32], ctypes.c_int),
         "gpe_prof_query": ([vp, i32, ctypes.POINTER(i64), dp, dp, ctypes.c_char_p], ctypes.c_int),
+        "gpe_tune_set": ([vp, i32, i32, i32], ctypes.c_int),
+        "gpe_tune_get": ([vp, i32] + [ctypes.POINTER(i32)] * 4, ctypes.c_int),
+        "gpe_autotune": ([vp, i32], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

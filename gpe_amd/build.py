@@ -23,16 +23,17 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
 ]
-UNITS = [
-    ("gpe_fp64.hip", ["-ffp-contract=off"]),
-    ("gpe_fp32.hip", ["-ffp-contract=fast"]),
-    ("gpe_capi.cpp", ["-ffp-contract=off", "-x", "hip"]),
-]
+N_VARIANTS = 5  # kTiles in csrc/launch.h
+UNITS = [("gpe_misc.hip", "gpe_misc", ["-ffp-contract=off"]),
+         ("gpe_capi.cpp", "gpe_capi", ["-ffp-contract=off", "-x", "hip"])]
+for _v in range(N_VARIANTS):
+    UNITS.append(("gpe_stage.hip", "stage_f64_v%d" % _v, ["-ffp-contract=off", "-DGPE_F64", "-DGPE_VID=%d" % _v]))
+    UNITS.append(("gpe_stage.hip", "stage_f32_v%d" % _v, ["-ffp-contract=fast", "-DGPE_VID=%d" % _v]))
 
 
 def _compile(unit):
-    src, extra = unit
-    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    src, name, extra = unit
+    out = os.path.join(OBJ, name + ".o")
     cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -42,7 +43,7 @@ def _compile(unit):
 
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=len(UNITS)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(UNITS), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(_compile, UNITS))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs

@@ -105,7 +105,10 @@ struct gpe_ctx {
   bool uploaded = false;
   gpe_step_state st{};
   StageArgs base{};
-  int zc_override = 0;
+  // launch configuration per stage kind: tile variant (index into kTiles)
+  // and z-chunk length (0 = automatic)
+  int vid[K_COUNT];
+  int zc[K_COUNT];
   // profiling
   bool prof = false;
   struct Ev {
@@ -120,13 +123,12 @@ struct gpe_ctx {
 
 namespace {
 
-int grid_for(gpe_ctx *c, int kind, StageArgs &A) {
-  Tile t = tile_shape((int)c->esz == 16 ? 16 : 8, c->R, kind);
+int grid_for_v(gpe_ctx *c, int vid, int zc, StageArgs &A) {
+  const Tile t = kTiles[vid];
   const int TY = t.nw * t.p;
   A.ntx = (c->P.nx + 63) / 64;
   A.nty = (c->P.ny + TY - 1) / TY;
   const int tiles2d = A.ntx * A.nty;
-  int zc = c->zc_override;
   if (zc <= 0) {
     // enough workgroups to fill 256 CUs several times over, but z-chunks
     // long enough that re-reading 2R planes per chunk stays small
@@ -134,12 +136,15 @@ int grid_for(gpe_ctx *c, int kind, StageArgs &A) {
     zc = (c->P.nz + nzc - 1) / nzc;
     if (zc < 32) zc = 32;
   }
+  if (zc < 8) zc = 8;
   if (zc > c->P.nz) zc = c->P.nz;
   A.zc = zc;
   A.nzc = (c->P.nz + zc - 1) / zc;
   A.nblocks = tiles2d * A.nzc;
   return A.nblocks;
 }
+
+int grid_for(gpe_ctx *c, int kind, StageArgs &A) { return grid_for_v(c, c->vid[kind], c->zc[kind], A); }
 
 hipEvent_t take_event(gpe_ctx *c) {
   if (!c->pool.empty()) {
@@ -175,11 +180,7 @@ int run_kind(gpe_ctx *c, int kind, StageArgs &A) {
     eb = take_event(c);
     if (ea) hipEventRecord(ea, c->stream);
   }
-  hipError_t e;
-  if (c->prec == GPE_FP64)
-    e = launch_stage_f64(kind, c->R, A, c->stream);
-  else
-    e = launch_stage_f32(kind, c->R, c->comp, A, c->stream);
+  hipError_t e = launch_stage((int)c->esz, c->vid[kind], kind, c->R, c->comp, A, c->stream);
   if (e != hipSuccess) return fail(GPE_EHIP, "stage %s launch: %s", kind_name(kind), hipGetErrorString(e));
   if (c->prof && ea && eb) {
     hipEventRecord(eb, c->stream);
@@ -531,11 +532,16 @@ int gpe_ctx_create(const gpe_params *params, gpe_ctx **out) {
   if (params->scheme == GPE_RK4 && (rc = alloc_field(c, &c->acc))) goto bad;
   if (params->scheme == GPE_RK45 && (rc = alloc_field(c, &c->scale))) goto bad;
   {
-    // partial buffers sized for the largest grid of any kind
-    int cap = 0;
+    // default launch configuration; partial buffers sized for the largest
+    // grid any variant and z-chunk can produce
     for (int kk = 0; kk < K_COUNT; ++kk) {
+      c->vid[kk] = 1;
+      c->zc[kk] = 0;
+    }
+    int cap = 0;
+    for (int v = 0; v < N_TILE_VARIANTS; ++v) {
       StageArgs tmp{};
-      int nb = grid_for(c, kk, tmp);
+      int nb = grid_for_v(c, v, 8, tmp);
       if (nb > cap) cap = nb;
     }
     if (cap < PW_BLOCKS) cap = PW_BLOCKS;
@@ -796,6 +802,103 @@ int gpe_solver_step(gpe_ctx *c, const void *var_in, int32_t halo, void *var_out,
   o.p = st->p;
   *st = o;
   if (c->prof) prof_drain(c);
+  return GPE_OK;
+}
+
+
+int gpe_tune_set(gpe_ctx *c, int32_t kind, int32_t variant, int32_t zchunk) {
+  if (!c) return fail(GPE_EINVAL, "NULL context");
+  if (kind < 0 || kind >= K_COUNT) return fail(GPE_EINVAL, "kind out of range");
+  if (variant < 0 || variant >= N_TILE_VARIANTS) return fail(GPE_EINVAL, "variant out of range");
+  if (zchunk < 0) return fail(GPE_EINVAL, "zchunk < 0");
+  c->vid[kind] = variant;
+  c->zc[kind] = zchunk;
+  return GPE_OK;
+}
+
+int gpe_tune_get(gpe_ctx *c, int32_t kind, int32_t *variant, int32_t *zchunk, int32_t *nw, int32_t *rows) {
+  if (!c) return fail(GPE_EINVAL, "NULL context");
+  if (kind < 0 || kind >= K_COUNT) return fail(GPE_EINVAL, "kind out of range");
+  if (variant) *variant = c->vid[kind];
+  if (zchunk) *zchunk = c->zc[kind];
+  if (nw) *nw = kTiles[c->vid[kind]].nw;
+  if (rows) *rows = kTiles[c->vid[kind]].p;
+  return GPE_OK;
+}
+
+// Time every tile variant and a few z-chunk lengths of each stage kind of
+// the context's scheme on the resident field, keep the fastest.  Outputs go
+// to the work fields (k, acc, next state), which every step rewrites before
+// reading, so the state is unchanged.
+int gpe_autotune(gpe_ctx *c, int32_t reps) {
+  if (!c) return fail(GPE_EINVAL, "NULL context");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  if (reps < 1) reps = 1;
+  std::vector<int> kinds;
+  switch (c->P.scheme) {
+    case GPE_EULER: kinds = {K_EULER}; break;
+    case GPE_RK2: kinds = {K_RK2_1, K_RK2_2}; break;
+    case GPE_RK4: kinds = {K_RK4_1, K_RK4_2, K_RK4_3, K_RK4_4}; break;
+    case GPE_RK45: kinds = {K_CK_1, K_CK_2, K_CK_3, K_CK_4, K_CK_5, K_CK_6}; break;
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const bool prof = c->prof;
+  c->prof = false;
+  int rc = GPE_OK;
+  for (int kind : kinds) {
+    StageArgs A = args_for(c);
+    A.sdtr = c->st.dt_re;
+    A.sdti = c->st.dt_im;
+    A.psi = c->psi;
+    A.lo = c->lo;
+    A.psi_out = c->psi2;
+    A.lo_out = c->lo2;
+    A.acc_in = c->acc;
+    A.acc_out = c->acc;
+    A.scale_in = c->scale;
+    A.scale_out = c->scale;
+    A.write_scale = 0;
+    for (int m = 0; m < 5; ++m) A.kin[m] = c->k[m] ? c->k[m] : c->psi;
+    // outputs into a k-field the kind does not read
+    int nin = (kind >= K_CK_2 && kind <= K_CK_6) ? kind - K_CK_1 : (kind == K_RK2_2 || kind == K_RK4_2 || kind == K_RK4_3 || kind == K_RK4_4) ? 1 : 0;
+    A.kout = c->k[nin < 5 ? nin : 0];
+    if (kind == K_RK4_2) A.kout = c->k[1];
+    if (kind == K_RK4_3) { A.kin[0] = c->k[1]; A.kout = c->k[0]; }
+    for (int m = 0; m < 5; ++m) A.b[m] = 0.1;
+    const int zcs[] = {0, 16, 32, 64, 128, 256};
+    float best = 1e30f;
+    int bv = c->vid[kind], bz = c->zc[kind];
+    for (int v = 0; v < N_TILE_VARIANTS && rc == GPE_OK; ++v) {
+      for (int zc : zcs) {
+        if (zc > c->P.nz) continue;
+        c->vid[kind] = v;
+        c->zc[kind] = zc;
+        if ((rc = run_kind(c, kind, A))) break;  // warm
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(GPE_EHIP, "event"); break; }
+        for (int r = 0; r < reps && rc == GPE_OK; ++r) rc = run_kind(c, kind, A);
+        if (rc) break;
+        hipEventRecord(e1, c->stream);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+          best = ms;
+          bv = v;
+          bz = zc;
+        }
+      }
+    }
+    c->vid[kind] = bv;
+    c->zc[kind] = bz;
+    if (rc) break;
+  }
+  c->prof = prof;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
   return GPE_OK;
 }
 

@@ -1,4 +1,4 @@
-// launch.h -- host-side view of the kernels in gpe_fp64.hip / gpe_fp32.hip.
+// launch.h -- host-side view of the kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -6,17 +6,20 @@
 
 namespace gpe {
 
-// Tile shape of a stage kernel: NW waves, P rows per wave (TY = NW*P rows
-// of 64 x-sites).  The launchers and the host grid computation both use
-// this table.
+// Tile shape of a stage kernel: NW waves, P rows per wave, so TY = NW*P rows
+// of 64 x-sites.  Each variant is compiled into its own translation unit
+// (gpe_stage.hip with -DGPE_VID) and the host picks one per stage kind
+// (defaults below, or the autotuner's choice).
 struct Tile {
   int nw, p;
 };
-Tile tile_shape(int prec_bytes, int radius, int kind);
+constexpr int N_TILE_VARIANTS = 5;
+constexpr Tile kTiles[N_TILE_VARIANTS] = {{4, 4}, {8, 2}, {16, 1}, {4, 2}, {8, 1}};
 
-// Launch stage kernel `kind` (radius 1 or 2) over A.nblocks workgroups.
-hipError_t launch_stage_f64(int kind, int radius, const StageArgs &A, hipStream_t s);
-hipError_t launch_stage_f32(int kind, int radius, bool comp, const StageArgs &A, hipStream_t s);
+// Launch stage kernel `kind` (radius 1 or 2) over A.nblocks workgroups with
+// tile variant `vid`.  comp selects the compensated FP32 form.
+hipError_t launch_stage(int prec_bytes, int vid, int kind, int radius, bool comp, const StageArgs &A,
+                        hipStream_t s);
 
 // Reduce per-block partials to out[0..2] (see reduce_partials).
 hipError_t launch_reduce(const double *pn, const double *pe, const double *pen, int n, double scale_norm,
@@ -30,5 +33,20 @@ hipError_t launch_norm_f32(const void *psi, const void *lo, int nx, int ny, int 
                            double w2, double w3, double *part, hipStream_t s);
 hipError_t launch_renorm_f64(void *psi, size_t n, double a, DivC b, hipStream_t s);
 hipError_t launch_renorm_f32(void *psi, void *lo, size_t n, double a, double b, hipStream_t s);
+
+// per-variant entry points (one definition per translation unit)
+#define GPE_DECL_VARIANT(P, V) \
+  hipError_t launch_stage_##P##_v##V(int kind, int radius, bool comp, const StageArgs &A, hipStream_t s);
+GPE_DECL_VARIANT(f64, 0)
+GPE_DECL_VARIANT(f64, 1)
+GPE_DECL_VARIANT(f64, 2)
+GPE_DECL_VARIANT(f64, 3)
+GPE_DECL_VARIANT(f64, 4)
+GPE_DECL_VARIANT(f32, 0)
+GPE_DECL_VARIANT(f32, 1)
+GPE_DECL_VARIANT(f32, 2)
+GPE_DECL_VARIANT(f32, 3)
+GPE_DECL_VARIANT(f32, 4)
+#undef GPE_DECL_VARIANT
 
 }  // namespace gpe

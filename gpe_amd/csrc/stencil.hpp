@@ -205,25 +205,57 @@ struct KindInfo {
                                                                                      : 0;
   static constexpr bool complex_coef = (KIND == K_RK2_2 || KIND == K_RK4_2 || KIND == K_RK4_3 || KIND == K_RK4_4);
   static constexpr bool final_ = (KIND == K_EULER || KIND == K_RK2_2 || KIND == K_RK4_4 || KIND == K_CK_6);
-  // FIFO entries (complex T) per point
-  static constexpr int nf = (KIND == K_RK2_2 || KIND == K_RK4_2) ? 1 : (KIND == K_RK4_4 || KIND == K_CK_6) ? 2 : 0;
+  static constexpr bool is_ck = (KIND >= K_CK_1 && KIND <= K_CK_6);
+  static constexpr bool has_acc = (KIND == K_RK4_4);
 };
+
+// Pointwise values the output plane needs from the fields read for its stage
+// input (complex T per point), kept in a register FIFO of R+1 planes.
+template <int KIND, bool COMP>
+struct FifoInfo {
+  static constexpr int nf = KIND == K_RK2_2   ? 1   // psi
+                            : KIND == K_RK4_2 ? 1   // kk1/6
+                            : KIND == K_RK4_4 ? 2   // psi, acc + kk3/3
+                            : KIND == K_CK_6  ? (COMP ? 3 : 2)  // psi_hi, sum c kk, sum dc kk | psi + sum c kk, sum dc kk
+                                              : 0;
+};
+
+// Pointwise fields read only at the output plane: the rk45 error scale and
+// the compensated low word.
+template <int KIND, bool COMP>
+struct OutInfo {
+  static constexpr int no = (KIND == K_CK_6 ? 1 : 0) + ((COMP && KindInfo<KIND>::final_) ? 1 : 0);
+};
+
+// Workgroup barrier that waits only for this wave's LDS traffic: the global
+// loads issued ahead for the next plane stay in flight across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 template <typename T, int R, int NW, int P, int KIND, bool COMP>
 __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
   constexpr int TX = 64, TY = NW * P, LX = TX + 2 * R, LY = TY + 2 * R;
   constexpr int NQ = 2 * R + 1;
+  constexpr int NT = 64 * NW;
   using KI = KindInfo<KIND>;
   constexpr int NIN = KI::nin;
-  constexpr int NF = KI::nf > 0 ? KI::nf : 1;
+  constexpr int NINA = NIN > 0 ? NIN : 1;
+  constexpr int NF = FifoInfo<KIND, COMP>::nf;
+  constexpr int NFA = NF > 0 ? NF : 1;
+  constexpr int NO = OutInfo<KIND, COMP>::no;
+  constexpr int NOA = NO > 0 ? NO : 1;
   constexpr bool IS_DBL = std::is_same<T, double>::value;
-  // FP32 rk45 final: psi_hi rides in F[0], the error partial in F[1] and the
-  // FP64 increment partial in FD.
-  constexpr bool FDBL = (!IS_DBL && KIND == K_CK_6);
+  constexpr int NYH = 2 * R * TX, NXH = 2 * R * TY, NH = NYH + NXH;
+  constexpr int HPT = (NH + NT - 1) / NT;
   using V = vec2_t<T>;
 
-  __shared__ V lds[2][LY][LX];
+  __shared__ V lds[2][LY * LX];
   __shared__ double red[3][NW];
+
+  struct Raw {
+    cx<T> ps;
+    cx<T> kk[NINA];
+    cx<T> ac;
+  };
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -247,118 +279,27 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
   const int nx = A.nx, ny = A.ny, nz = A.nz, per = A.periodic;
   const size_t plane = (size_t)nx * (size_t)ny;
 
+  // Own points: column gi, rows gj[p].  Loads use the boundary-mapped offset
+  // (always in range); stores happen only for points inside the grid.
   const int gi = tx0 + lane;
   const bool act_x = gi < nx;
   const int gim = bcmap(min(gi, nx - 1 + R), nx, per);
   int gj[P];
-  size_t roff[P];
+  int roff[P];
 #pragma unroll
   for (int p = 0; p < P; ++p) {
     gj[p] = ty0 + wave * P + p;
     const int gjm = bcmap(min(gj[p], ny - 1 + R), ny, per);
-    roff[p] = (size_t)gjm * nx + gim;
+    roff[p] = gjm * nx + gim;
   }
-
-  const cx<T> H = KIND == K_RK4_4 ? mk((T)A.dtr, (T)A.dti) : mk((T)A.hr, (T)A.hi);
-
-  // Stage input u at one site, plus (wantF) the pointwise partials for the
-  // FIFO.  Ghost cells need u only.
-  auto load_u = [&](size_t s, cx<T> &u, cx<T> *F, cx<double> &FD, const bool wantF) {
-    cx<T> ps = ldc<T>(A.psi, s);
-    if constexpr (NIN == 0) {
-      u = ps;
-    } else if constexpr (KI::complex_coef) {
-      cx<T> k0 = ldc<T>(A.kin[0], s);
-      u = ps + cmul(H, k0);
-      if (!wantF) return;
-      if constexpr (KIND == K_RK2_2) {
-        F[0] = ps;
-      } else if constexpr (KIND == K_RK4_2) {
-        F[0] = rdiv(k0, A.d6, 1.0 / 6.0);  // kk1/6
-      } else if constexpr (KIND == K_RK4_4) {
-        cx<T> ac = ldc<T>(A.acc_in, s);
-        F[0] = ps;
-        F[1] = ac + rdiv(k0, A.d3, 1.0 / 3.0);  // (kk1/6 + kk2/3) + kk3/3
-      }
-    } else {
-      cx<T> kk[NIN];
+  // Ghost cells this thread fills: fixed for the whole z march.
+  int hl[HPT], hoff[HPT];
 #pragma unroll
-      for (int m = 0; m < NIN; ++m) kk[m] = ldc<T>(A.kin[m], s);
-      u = ps;
-#pragma unroll
-      for (int m = 0; m < NIN; ++m) u = u + rmul((T)A.b[m], kk[m]);
-      if (!wantF) return;
-      if constexpr (KIND == K_CK_6) {
-        if constexpr (IS_DBL) {
-          cx<T> pn = ps;
-#pragma unroll
-          for (int m = 0; m < 5; ++m) pn = pn + rmul((T)A.c[m + 1], kk[m]);
-          cx<T> pe = rmul((T)A.dc[1], kk[0]);
-#pragma unroll
-          for (int m = 1; m < 5; ++m) pe = pe + rmul((T)A.dc[m + 1], kk[m]);
-          F[0] = pn;
-          F[1] = pe;
-        } else {
-          cx<double> pd = rmul(A.c[1], todbl(kk[0]));
-          pd = pd + rmul(A.c[3], todbl(kk[2]));
-          pd = pd + rmul(A.c[4], todbl(kk[3]));
-          cx<T> pe = rmul((T)A.dc[1], kk[0]);
-#pragma unroll
-          for (int m = 1; m < 5; ++m) pe = pe + rmul((T)A.dc[m + 1], kk[m]);
-          F[0] = ps;
-          F[1] = pe;
-          FD = pd;
-        }
-      }
-    }
-  };
-
-  cx<T> q[NQ][P];
-  cx<T> F[R + 1][P][NF];
-  cx<double> FD[R + 1][P];
-
-  // Prime the queue with planes z0-R .. z0+R-1.
-#pragma unroll
-  for (int m = 0; m < 2 * R; ++m) {
-    const int zp = bcmap(z0 - R + m, nz, per);
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-      cx<T> Ft[NF];
-      cx<double> FDt;
-      load_u((size_t)zp * plane + roff[p], q[m][p], Ft, FDt, m >= R);
-      if (m >= R) {
-        if constexpr (KI::nf > 0) {
-#pragma unroll
-          for (int f = 0; f < NF; ++f) F[m - R][p][f] = Ft[f];
-        }
-        if constexpr (FDBL) FD[m - R][p] = FDt;
-      }
-    }
-  }
-
-  double acc_norm = 0.0, acc_emax = 0.0, acc_en = 0.0;
-  const int nthreads = 64 * NW;
-  constexpr int NYH = 2 * R * TX, NXH = 2 * R * TY, NH = NYH + NXH;
-
-  for (int k = z0; k < z1; ++k) {
-    const int buf = (k - z0) & 1;
-    // new plane k+R into the queue tail and FIFO tail
-    {
-      const int zp = bcmap(k + R, nz, per);
-#pragma unroll
-      for (int p = 0; p < P; ++p) {
-        cx<T> Ft[NF];
-        cx<double> FDt;
-        load_u((size_t)zp * plane + roff[p], q[2 * R][p], Ft, FDt, true);
-        if constexpr (KI::nf > 0) {
-#pragma unroll
-          for (int f = 0; f < NF; ++f) F[R][p][f] = Ft[f];
-        }
-        if constexpr (FDBL) FD[R][p] = FDt;
-      }
-    }
-    // ghost cells of plane k
-    for (int h = threadIdx.x; h < NH; h += nthreads) {
+  for (int i = 0; i < HPT; ++i) {
+    const int h = threadIdx.x + i * NT;
+    hl[i] = -1;
+    hoff[i] = 0;
+    if (h < NH) {
       int lr, lc;
       if (h < NYH) {
         const int rs = h / TX, c = h - rs * TX;
@@ -370,52 +311,173 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
         lr = r + R;
         lc = cs < R ? cs : TX + cs;
       }
+      hl[i] = lr * LX + lc;
       const int hgi = bcmap(min(tx0 + lc - R, nx - 1 + R), nx, per);
       const int hgj = bcmap(min(ty0 + lr - R, ny - 1 + R), ny, per);
-      cx<T> u, Ft[NF];
-      cx<double> FDt;
-      load_u((size_t)k * plane + (size_t)hgj * nx + hgi, u, Ft, FDt, false);
-      V v;
-      v.x = u.re;
-      v.y = u.im;
-      lds[buf][lr][lc] = v;
+      hoff[i] = hgj * nx + hgi;
     }
-    // plane k interior from the queue centre
+  }
+
+  const cx<T> H = KIND == K_RK4_4 ? mk((T)A.dtr, (T)A.dti) : mk((T)A.hr, (T)A.hi);
+
+  auto load_raw = [&](size_t s, Raw &r, const bool full) {
+    r.ps = ldc<T>(A.psi, s);
+    if constexpr (NIN > 0) {
+#pragma unroll
+      for (int m = 0; m < NIN; ++m) r.kk[m] = ldc<T>(A.kin[m], s);
+    }
+    if constexpr (KI::has_acc) {
+      if (full) r.ac = ldc<T>(A.acc_in, s);
+    }
+  };
+  // stage input u = psi + sum coef * k
+  auto make_u = [&](const Raw &r) -> cx<T> {
+    if constexpr (NIN == 0) {
+      return r.ps;
+    } else if constexpr (KI::complex_coef) {
+      return r.ps + cmul(H, r.kk[0]);
+    } else {
+      cx<T> u = r.ps;
+#pragma unroll
+      for (int m = 0; m < NIN; ++m) u = u + rmul((T)A.b[m], r.kk[m]);
+      return u;
+    }
+  };
+  auto make_F = [&](const Raw &r, cx<T> *F) {
+    if constexpr (KIND == K_RK2_2) {
+      F[0] = r.ps;
+    } else if constexpr (KIND == K_RK4_2) {
+      F[0] = rdiv(r.kk[0], A.d6, 1.0 / 6.0);  // kk1/6
+    } else if constexpr (KIND == K_RK4_4) {
+      F[0] = r.ps;
+      F[1] = r.ac + rdiv(r.kk[0], A.d3, 1.0 / 3.0);  // (kk1/6 + kk2/3) + kk3/3
+    } else if constexpr (KIND == K_CK_6) {
+      cx<T> pe = rmul((T)A.dc[1], r.kk[0]);
+#pragma unroll
+      for (int m = 1; m < 5; ++m) pe = pe + rmul((T)A.dc[m + 1], r.kk[m]);
+      if constexpr (COMP) {
+        cx<T> pd = rmul((T)A.c[1], r.kk[0]);  // c2 = c5 = 0
+        pd = pd + rmul((T)A.c[3], r.kk[2]);
+        pd = pd + rmul((T)A.c[4], r.kk[3]);
+        F[0] = r.ps;
+        F[1] = pd;
+        F[2] = pe;
+      } else {
+        cx<T> pn = r.ps;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) pn = pn + rmul((T)A.c[m + 1], r.kk[m]);
+        F[0] = pn;
+        F[1] = pe;
+      }
+    }
+  };
+  auto load_out = [&](size_t s, cx<T> *o) {
+    int i = 0;
+    if constexpr (KIND == K_CK_6) o[i++] = ldc<T>(A.scale_in, s);
+    if constexpr (COMP && KI::final_) o[i++] = ldc<T>(A.lo, s);
+    (void)i;
+  };
+
+  cx<T> q[NQ][P];
+  cx<T> F[R + 1][P][NFA];
+
+  // Prime the queue with planes z0-R .. z0+R-1 (and the FIFO with z0 ..).
+#pragma unroll
+  for (int m = 0; m < 2 * R; ++m) {
+    const size_t zoff = (size_t)bcmap(z0 - R + m, nz, per) * plane;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      Raw r;
+      load_raw(zoff + roff[p], r, m >= R);
+      q[m][p] = make_u(r);
+      if constexpr (NF > 0) {
+        if (m >= R) make_F(r, F[m - R][p]);
+      }
+    }
+  }
+
+  // Loads for plane k: interior of k+R, ghosts of k, output-time fields of k.
+  Raw rn[P];
+  Raw hn[HPT];
+  cx<T> on[P][NOA];
+  auto issue = [&](int k) {
+    const size_t zi = (size_t)bcmap(k + R, nz, per) * plane;
+    const size_t zk = (size_t)k * plane;
+#pragma unroll
+    for (int p = 0; p < P; ++p) load_raw(zi + roff[p], rn[p], true);
+#pragma unroll
+    for (int i = 0; i < HPT; ++i)
+      if (hl[i] >= 0) load_raw(zk + hoff[i], hn[i], false);
+    if constexpr (NO > 0) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) load_out(zk + roff[p], on[p]);
+    }
+  };
+  issue(z0);
+
+  double acc_norm = 0.0, acc_emax = 0.0, acc_en = 0.0;
+
+  for (int k = z0; k < z1; ++k) {
+    V *Ld = lds[(k - z0) & 1];
+    // consume what was prefetched for plane k
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      q[2 * R][p] = make_u(rn[p]);
+      if constexpr (NF > 0) make_F(rn[p], F[R][p]);
+    }
+    cx<T> hu[HPT];
+#pragma unroll
+    for (int i = 0; i < HPT; ++i) hu[i] = hl[i] >= 0 ? make_u(hn[i]) : mk((T)0, (T)0);
+    cx<T> oc[P][NOA];
+    if constexpr (NO > 0) {
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int o = 0; o < NO; ++o) oc[p][o] = on[p][o];
+    }
+    // prefetch plane k+1 while plane k is computed
+    if (k + 1 < z1) issue(k + 1);
+    // stage input of plane k into LDS: ghosts and interior
+#pragma unroll
+    for (int i = 0; i < HPT; ++i) {
+      if (hl[i] >= 0) {
+        V v;
+        v.x = hu[i].re;
+        v.y = hu[i].im;
+        Ld[hl[i]] = v;
+      }
+    }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       V v;
       v.x = q[R][p].re;
       v.y = q[R][p].im;
-      lds[buf][wave * P + p + R][lane + R] = v;
+      Ld[(wave * P + p + R) * LX + lane + R] = v;
     }
-    __syncthreads();
+    lds_barrier();
 
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-      const int lr = wave * P + p + R, lc = lane + R;
-      auto L2 = [&](int r, int c) {
-        V v = lds[buf][r][c];
+      const int c = (wave * P + p + R) * LX + lane + R;
+      auto LD = [&](int off) {
+        V v = Ld[c + off];
         return mk<T>(v.x, v.y);
       };
       const cx<T> c0 = q[R][p];
-      cx<T> xm1 = L2(lr, lc - 1), xp1 = L2(lr, lc + 1), ym1 = L2(lr - 1, lc), yp1 = L2(lr + 1, lc);
-      cx<T> xm2 = xm1, xp2 = xp1, ym2 = ym1, yp2 = yp1, zm2, zp2;
+      cx<T> xm1 = LD(-1), xp1 = LD(1), ym1 = LD(-LX), yp1 = LD(LX);
+      cx<T> xm2 = xm1, xp2 = xp1, ym2 = ym1, yp2 = yp1;
+      const cx<T> zm2 = q[0][p], zp2 = q[2 * R][p];
       if constexpr (R == 2) {
-        xm2 = L2(lr, lc - 2);
-        xp2 = L2(lr, lc + 2);
-        ym2 = L2(lr - 2, lc);
-        yp2 = L2(lr + 2, lc);
-        zm2 = q[0][p];
-        zp2 = q[4][p];
-      } else {
-        zm2 = q[0][p];
-        zp2 = q[2][p];
+        xm2 = LD(-2);
+        xp2 = LD(2);
+        ym2 = LD(-2 * LX);
+        yp2 = LD(2 * LX);
       }
       const cx<T> zm1 = q[R - 1][p], zp1 = q[R + 1][p];
-      cx<T> dxx = d2<T, R>(xm2, xm1, c0, xp1, xp2, A.d_xx, A.inv_xx);
-      cx<T> dyy = d2<T, R>(ym2, ym1, c0, yp1, yp2, A.d_yy, A.inv_yy);
-      cx<T> dzz = d2<T, R>(zm2, zm1, c0, zp1, zp2, A.d_zz, A.inv_zz);
-      cx<T> lap = (dxx + dyy) + dzz;
+      const cx<T> dxx = d2<T, R>(xm2, xm1, c0, xp1, xp2, A.d_xx, A.inv_xx);
+      const cx<T> dyy = d2<T, R>(ym2, ym1, c0, yp1, yp2, A.d_yy, A.inv_yy);
+      const cx<T> dzz = d2<T, R>(zm2, zm1, c0, zp1, zp2, A.d_zz, A.inv_zz);
+      const cx<T> lap = (dxx + dyy) + dzz;
       const bool act = act_x && (gj[p] < ny);
       const size_t s = (size_t)k * plane + (size_t)gj[p] * nx + gi;
 
@@ -436,11 +498,12 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
           } else if constexpr (KIND == K_RK4_2) {
             stc<T>(A.kout, s, f);
             stc<T>(A.acc_out, s, F[0][p][0] + rdiv(f, A.d3, 1.0 / 3.0));
-          } else if constexpr (KIND >= K_CK_1 && KIND <= K_CK_6) {
-            cx<T> kk;
-            if constexpr (IS_DBL) {
+          } else if constexpr (KI::is_ck) {
+            cx<T> kk;  // kk = dt*f
+            if constexpr (IS_DBL || !COMP) {
               kk = cmul(mk((T)A.dtr, (T)A.dti), f);
             } else {
+              // FP64 dt: no systematic bias from dt rounded to FP32
               cx<double> kd = cmul(mk(A.dtr, A.dti), todbl(f));
               kk = mk((T)kd.re, (T)kd.im);
             }
@@ -455,60 +518,74 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
             } else if constexpr (KIND < K_CK_6) {
               stc<T>(A.kout, s, kk);
             } else {
-              // K_CK_6: new state, error norm
-              const cx<T> sc = ldc<T>(A.scale_in, s);
-              cx<T> err = F[0][p][1] + rmul((T)A.dc[6], kk);
+              // K_CK_6: new state and the error norm (solve.f90:365-375, 184)
+              const cx<T> sc = oc[p][0];
+              const cx<T> err = F[0][p][NF - 1] + rmul((T)A.dc[6], kk);
               cx<double> np;
               if constexpr (IS_DBL) {
                 np = F[0][p][0] + rmul((T)A.c[6], kk);
                 stc<T>(A.psi_out, s, np);
-                cx<double> q2 = cdiv(err, sc);
-                double e = cabs_(q2);
+                const double e = cabs_(cdiv(err, sc));
                 acc_emax = (e > acc_emax || e != e) ? e : acc_emax;
               } else {
-                cx<double> dl = FD[0][p] + rmul(A.c[6], todbl(kk));
-                cx<double> base = todbl(F[0][p][0]);
-                if constexpr (COMP) base = base + todbl(ldc<T>(A.lo, s));
-                np = base + dl;
-                cx<T> hi = mk((T)np.re, (T)np.im);
-                stc<T>(A.psi_out, s, hi);
-                if constexpr (COMP) stc<T>(A.lo_out, s, mk((T)(np.re - (double)hi.re), (T)(np.im - (double)hi.im)));
-                float e = sqrtf(err.re * err.re + err.im * err.im) / sqrtf(sc.re * sc.re + sc.im * sc.im);
+                if constexpr (COMP) {
+                  const cx<double> dl = todbl(F[0][p][1]) + rmul(A.c[6], todbl(kk));
+                  np = (todbl(F[0][p][0]) + todbl(oc[p][1])) + dl;
+                  const cx<T> hi = mk((T)np.re, (T)np.im);
+                  stc<T>(A.psi_out, s, hi);
+                  stc<T>(A.lo_out, s, mk((T)(np.re - (double)hi.re), (T)(np.im - (double)hi.im)));
+                } else {
+                  const cx<T> nf = F[0][p][0] + rmul((T)A.c[6], kk);
+                  stc<T>(A.psi_out, s, nf);
+                  np = todbl(nf);
+                }
+                const float e = sqrtf(err.re * err.re + err.im * err.im) / sqrtf(sc.re * sc.re + sc.im * sc.im);
                 acc_emax = ((double)e > acc_emax || e != e) ? (double)e : acc_emax;
               }
               const double wn = nweight(gi, nx, A.w1, A.w2, A.w3) * nweight(gj[p], ny, A.w1, A.w2, A.w3) *
                                 nweight(k, nz, A.w1, A.w2, A.w3);
-              acc_norm += wn * (np.re * np.re + np.im * np.im);
+              if constexpr (IS_DBL) {
+                const double mm = cabs_(np);
+                acc_norm += wn * (mm * mm);
+              } else {
+                acc_norm += wn * (np.re * np.re + np.im * np.im);
+              }
             }
           } else if constexpr (KI::final_) {
             // K_EULER, K_RK2_2, K_RK4_4
-            cx<T> ps;
-            cx<T> incr_t;  // the complex the final dt multiplies
+            cx<T> ps, incr;  // psi' = ps + dt*incr
             if constexpr (KIND == K_EULER) {
               ps = c0;
-              incr_t = f;
+              incr = f;
             } else if constexpr (KIND == K_RK2_2) {
               ps = F[0][p][0];
-              incr_t = f;
+              incr = f;
             } else {
               ps = F[0][p][0];
-              incr_t = F[0][p][1] + rdiv(f, A.d6, 1.0 / 6.0);
+              incr = F[0][p][1] + rdiv(f, A.d6, 1.0 / 6.0);
             }
             cx<double> np;
             if constexpr (IS_DBL) {
-              np = ps + cmul(mk((T)A.dtr, (T)A.dti), incr_t);
+              np = ps + cmul(mk((T)A.dtr, (T)A.dti), incr);
               stc<T>(A.psi_out, s, np);
-            } else {
-              cx<double> base = todbl(ps);
-              if constexpr (COMP) base = base + todbl(ldc<T>(A.lo, s));
-              np = base + cmul(mk(A.dtr, A.dti), todbl(incr_t));
-              cx<T> hi = mk((T)np.re, (T)np.im);
+            } else if constexpr (COMP) {
+              np = (todbl(ps) + todbl(oc[p][0])) + cmul(mk(A.dtr, A.dti), todbl(incr));
+              const cx<T> hi = mk((T)np.re, (T)np.im);
               stc<T>(A.psi_out, s, hi);
-              if constexpr (COMP) stc<T>(A.lo_out, s, mk((T)(np.re - (double)hi.re), (T)(np.im - (double)hi.im)));
+              stc<T>(A.lo_out, s, mk((T)(np.re - (double)hi.re), (T)(np.im - (double)hi.im)));
+            } else {
+              const cx<T> nf = ps + cmul(mk((T)A.dtr, (T)A.dti), incr);
+              stc<T>(A.psi_out, s, nf);
+              np = todbl(nf);
             }
             const double wn = nweight(gi, nx, A.w1, A.w2, A.w3) * nweight(gj[p], ny, A.w1, A.w2, A.w3) *
                               nweight(k, nz, A.w1, A.w2, A.w3);
-            acc_norm += wn * (np.re * np.re + np.im * np.im);
+            if constexpr (IS_DBL) {
+              const double mm = cabs_(np);
+              acc_norm += wn * (mm * mm);
+            } else {
+              acc_norm += wn * (np.re * np.re + np.im * np.im);
+            }
           }
         }
       }
@@ -519,16 +596,14 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
     for (int m = 0; m < 2 * R; ++m)
 #pragma unroll
       for (int p = 0; p < P; ++p) q[m][p] = q[m + 1][p];
+    if constexpr (NF > 0) {
 #pragma unroll
-    for (int m = 0; m < R; ++m)
+      for (int m = 0; m < R; ++m)
 #pragma unroll
-      for (int p = 0; p < P; ++p) {
-        if constexpr (KI::nf > 0) {
+        for (int p = 0; p < P; ++p)
 #pragma unroll
           for (int f = 0; f < NF; ++f) F[m][p][f] = F[m + 1][p][f];
-        }
-        if constexpr (FDBL) FD[m][p] = FD[m + 1][p];
-      }
+    }
   }
 
   // Deterministic block reductions: xor-shuffle within the wave, then the
@@ -562,10 +637,11 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
 }
 
 // Sum / max of the per-block partials in a fixed order (one workgroup).
-// out[0] = sum(part_norm) * scale_norm, out[1] = max(part_emax) / eps,
-// out[2] = sum(part_energy) * scale_energy; hist (if set) gets out[0].
+// out[0] = sum(part_norm) * scale_norm, out[1] = max(part_emax) / eps
+// (maxval(...)/eps, solve.f90:184), out[2] = sum(part_energy) * scale_energy;
+// hist (if set) gets out[0].
 static __global__ void __launch_bounds__(256) reduce_partials(const double *pn, const double *pe, const double *pen, int n,
-                                                       double scale_norm, double inv_eps, double scale_energy,
+                                                       double scale_norm, double eps, double scale_energy,
                                                        double *out, double *hist) {
   __shared__ double s0[256], s1[256], s2[256];
   double a = 0.0, m = 0.0, c = 0.0;
@@ -589,7 +665,7 @@ static __global__ void __launch_bounds__(256) reduce_partials(const double *pn, 
   }
   if (threadIdx.x == 0) {
     out[0] = s0[0] * scale_norm;
-    out[1] = s1[0] * inv_eps;
+    out[1] = s1[0] / eps;
     out[2] = s2[0] * scale_energy;
     if (hist) *hist = out[0];
   }

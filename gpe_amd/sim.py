@@ -169,3 +169,22 @@ class Simulation:
             if n.value:
                 out[nm] = (n.value, ms.value, by.value)
         return out
+
+    # -- launch configuration -----------------------------------------------
+    def tune_set(self, kind, variant, zchunk=0):
+        k = _lib.KIND_NAMES.index(kind) if isinstance(kind, str) else int(kind)
+        _lib.check(self._L.gpe_tune_set(self._ctx, k, int(variant), int(zchunk)))
+
+    def tune_get(self):
+        """{kind name: (variant, zchunk, waves, rows per wave)}"""
+        out = {}
+        for k, nm in enumerate(_lib.KIND_NAMES):
+            v, z, w, r = (ctypes.c_int32() for _ in range(4))
+            _lib.check(self._L.gpe_tune_get(self._ctx, k, ctypes.byref(v), ctypes.byref(z), ctypes.byref(w),
+                                            ctypes.byref(r)))
+            out[nm] = (v.value, z.value, w.value, r.value)
+        return out
+
+    def autotune(self, reps=3):
+        _lib.check(self._L.gpe_autotune(self._ctx, int(reps)))
+        return self.tune_get()

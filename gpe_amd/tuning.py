@@ -1,0 +1,54 @@
+"""Persisted launch configurations of the stage kernels.
+
+The autotuner (gpe_autotune in the C ABI) times every compiled tile variant
+and several z-chunk lengths per stage kind on the resident field.  Results
+are kept in a JSON file keyed by (grid, order, precision, scheme) so a later
+run of the same problem skips the search.
+"""
+import json
+import os
+
+from . import _lib
+
+DEFAULT_PATH = os.environ.get("GPE_TUNING_FILE", os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                              "tuning.json"))
+
+
+def key(cfg):
+    return "%dx%dx%d/o%d/%s/%s/b%d" % (cfg.nx, cfg.ny, cfg.nz, cfg.order, cfg.precision, cfg.scheme, cfg.bcs)
+
+
+def load(path=DEFAULT_PATH):
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        return json.load(fh)
+
+
+def save(table, path=DEFAULT_PATH):
+    tmp = path + ".tmp"
+    with open(tmp, "w") as fh:
+        json.dump(table, fh, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def apply(sim, path=DEFAULT_PATH):
+    """Apply a stored configuration for sim's problem; True if one existed."""
+    ent = load(path).get(key(sim.cfg))
+    if not ent:
+        return False
+    for kind, (variant, zc) in ent.items():
+        if kind in _lib.KIND_NAMES:
+            sim.tune_set(kind, variant, zc)
+    return True
+
+
+def tune(sim, reps=3, path=DEFAULT_PATH, store=True):
+    """Autotune sim's stage kernels (state unchanged) and persist the result."""
+    got = sim.autotune(reps)
+    ent = {k: [v[0], v[1]] for k, v in got.items()}
+    if store:
+        table = load(path)
+        table[key(sim.cfg)] = ent
+        save(table, path)
+    return ent

@@ -122,6 +122,19 @@ int gpe_sim_renormalise(gpe_ctx *ctx, double nn, double norm);
 /* f(psi) of the resident field into a host buffer (get_rhs, solve.f90:382). */
 int gpe_sim_rhs(gpe_ctx *ctx, void *out, int32_t host_type);
 
+/* Launch configuration of the stage kernels.  Each stage kind (0 ..
+ * GPE_PROF_KINDS-1, see gpe_prof_query for names) runs with one of several
+ * compiled tile variants (waves x rows per wave) and a z-chunk length
+ * (0 = automatic).  No reference counterpart: the reference has no device
+ * kernels to configure. */
+int gpe_tune_set(gpe_ctx *ctx, int32_t kind, int32_t variant, int32_t zchunk);
+int gpe_tune_get(gpe_ctx *ctx, int32_t kind, int32_t *variant, int32_t *zchunk, int32_t *waves,
+                 int32_t *rows_per_wave);
+/* Time every variant and several z-chunks of each stage kind of the
+ * context's scheme on the resident field (reps launches each) and keep the
+ * fastest.  The state is unchanged. */
+int gpe_autotune(gpe_ctx *ctx, int32_t reps);
+
 /* Kernel timing with HIP events recorded on the context's stream around
  * every stage launch.  enable=1 resets the counters. */
 int gpe_prof_enable(gpe_ctx *ctx, int32_t enable);
