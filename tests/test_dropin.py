@@ -1,0 +1,87 @@
+"""The Fortran drop-in: the reference's parameters/error modules + the
+drop-in module solve (gpe_amd/fortran/solve.f90) + a driver running the
+gpe.f90 time loop, against the oracle.  Binaries are built by
+`make -C oracle ref` (they link the reference's own modules)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from gpe_amd import ic
+from gpe_amd.config import RunConfig
+from gpe_amd.state_io import read_state, write_run_in, write_state
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN_D = os.path.join(ROOT, "oracle", "_ref", "gpe_dropin_d")
+BIN_S = os.path.join(ROOT, "oracle", "_ref", "gpe_dropin_s")
+need_bins = pytest.mark.skipif(not (os.path.exists(BIN_D) and os.path.exists(BIN_S)),
+                               reason="drop-in harness not built (make -C oracle ref)")
+
+
+def cfg_for(**kw):
+    base = dict(nx=24, ny=20, nz=18, xr=4.6, yr=3.8, zr=3.4, eqn=4, g=1.0, mu=1.0, omx=0.3, omy=0.2,
+                omz=0.4, bcs=2, order=4, scheme="rk4", tau=0.01)
+    base.update(kw)
+    return RunConfig(**base)
+
+
+def run_dropin(tmp_path, exe, cfg, psi, nsteps):
+    # end_time = (N - 1/2) dt gives exactly N steps of the loop t+im_t <= end_time
+    write_run_in(str(tmp_path / "run.in"), cfg, (nsteps - 0.5) * cfg.tau)
+    write_state(str(tmp_path / "end_state0000.dat"), psi, dt=cfg.dt)
+    open(tmp_path / "RUNNING", "w").close()
+    r = subprocess.run([exe, "run.in", "end_state0000.dat", "out.dat"], cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=120)
+    return r
+
+
+@need_bins
+def test_dropin_unknown_scheme_emergency_stop(tmp_path):
+    # CPU-side failure path: the drop-in stops through the reference's
+    # emergency_stop (error.f90:24-46) before touching the GPU
+    cfg = cfg_for()
+    cfg.scheme = "rk3"  # not a scheme the reference knows
+    psi = ic.vortex_ring(cfg, 1.5)
+    r = run_dropin(tmp_path, BIN_D, cfg, psi, 2)
+    assert (tmp_path / "ERROR").exists()
+    assert "Unrecognised time stepping scheme" in (tmp_path / "ERROR").read_text()
+    assert not (tmp_path / "RUNNING").exists()
+    assert not (tmp_path / "out.dat").exists()
+
+
+@pytest.mark.gpu
+@need_bins
+@pytest.mark.parametrize("scheme", ["euler", "rk2", "rk4", "rk45"])
+@pytest.mark.parametrize("bcs", [1, 2])
+def test_dropin_fp64_matches_oracle(tmp_path, oracle, scheme, bcs):
+    cfg = cfg_for(scheme=scheme, bcs=bcs, tau=0.002 if scheme == "euler" else 0.01)
+    psi = ic.vortex_ring(cfg, 1.5) * (1 + 0.02 * np.random.default_rng(3).standard_normal(cfg.shape))
+    n = 8
+    print("[case] drop-in fp64", scheme, bcs, flush=True)
+    r = run_dropin(tmp_path, BIN_D, cfg, psi, n)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got, p, t, im_t, dt = read_state(str(tmp_path / "out.dat"))
+    st = oracle.make_state(cfg.dt)
+    want, _ = oracle.run(cfg, st, psi, n)
+    assert p == n
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12
+    assert t == pytest.approx(st.t, rel=1e-9)
+    if scheme == "rk45":
+        assert (tmp_path / "timestep.dat").exists()
+        assert len((tmp_path / "timestep.dat").read_text().splitlines()) == n
+
+
+@pytest.mark.gpu
+@need_bins
+def test_dropin_fp32_tracks_single_precision_oracle(tmp_path, oracle):
+    cfg = cfg_for(scheme="rk4", precision="fp32")
+    psi = ic.vortex_ring(cfg, 1.5).astype(np.complex64)
+    n = 8
+    print("[case] drop-in fp32", flush=True)
+    r = run_dropin(tmp_path, BIN_S, cfg, psi, n)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got, p, t, im_t, dt = read_state(str(tmp_path / "out.dat"), single=True)
+    want, _ = oracle.run(cfg, oracle.make_state(cfg.dt), psi, n)
+    assert p == n
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5
