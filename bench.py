@@ -43,10 +43,17 @@ def workload(precision, n=512):
                      eqn=4, g=1.0, mu=1.0, tau=0.05, precision=precision)
 
 
-def run_path(precision, psi, steps, warmup):
+def run_path(precision, psi, steps, warmup, tune=True):
+    from gpe_amd import tuning
     from gpe_amd.sim import Simulation
     cfg = workload(precision)
     sim = Simulation(cfg, psi)
+    if tune:
+        # launch configuration from the autotuner (setup, not timed)
+        if not tuning.apply(sim):
+            tuning.tune(sim, reps=3)
+        sim.set_state(cfg.dt)
+        sim.upload(psi)
     log(precision, "warmup", warmup)
     sim.advance(warmup)
     s0 = sim.state()
@@ -62,6 +69,7 @@ def run_path(precision, psi, steps, warmup):
     sim.advance(max(2, min(steps, 10)))
     prof = sim.prof_query()
     s2 = sim.state()
+    tiles = {k: v for k, v in sim.tune_get().items() if k in prof}
     sim.close()
     sites = cfg.nx * cfg.ny * cfg.nz
     return {
@@ -73,6 +81,8 @@ def run_path(precision, psi, steps, warmup):
         "prof": prof,
         "rejects_last": s2["n_rejects"],
         "sites": sites,
+        "tiles": tiles,
+        "attempts_per_step": prof.get("ck_6", (0,))[0] / max(2, min(steps, 10)),
     }
 
 
@@ -122,7 +132,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    from gpe_amd import ic
+    from gpe_amd import _lib, ic
+    probe = _lib.probe_bandwidth(1 << 31, 20)
+    log("HBM probe GB/s: read %.0f write %.0f copy %.0f" % (probe["read"], probe["write"], probe["copy"]))
     t0 = time.perf_counter()
     psi = ic.random_phase(workload("fp64"), a.seed)
     log("IC ready in %.1f s" % (time.perf_counter() - t0))
@@ -158,6 +170,10 @@ def main():
         "fp64": {"value": round(r64["su_per_s"], 1), "ms_per_step": round(r64["ms_per_step"], 4),
                  "roofline": roofline_of(r64)},
         "fp32c_over_fp64": round(r32["su_per_s"] / r64["su_per_s"], 3),
+        "hbm_probe_gbs": {k: round(v, 1) for k, v in probe.items()},
+        "sim_time": {"fp32c": round(r32["t_sim"], 6), "fp64": round(r64["t_sim"], 6)},
+        "attempts_per_step": {"fp32c": r32["attempts_per_step"], "fp64": r64["attempts_per_step"]},
+        "tiles": {"fp32c": r32["tiles"], "fp64": r64["tiles"]},
         "step_gbs": {"fp32c": step_roofline(r32, 232.0), "fp64": step_roofline(r64, 432.0)},
         "kernels": {"fp32c": {k: [v[0], round(v[1], 3)] for k, v in r32["prof"].items()},
                     "fp64": {k: [v[0], round(v[1], 3)] for k, v in r64["prof"].items()}},

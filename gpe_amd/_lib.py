@@ -7,7 +7,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgpehip.so")
+LIB_PATH = os.environ.get("GPE_LIBRARY") or os.path.join(HERE, "libgpehip.so")
 
 GPE_OK = 0
 GPE_EULER, GPE_RK2, GPE_RK4, GPE_RK45 = 1, 2, 4, 45
@@ -28,7 +28,7 @@ EXPORTS = [
     "gpe_sim_upload", "gpe_sim_download", "gpe_sim_set_state", "gpe_sim_get_state",
     "gpe_sim_advance", "gpe_sim_norm_history", "gpe_sim_norm", "gpe_sim_energy",
     "gpe_sim_mass", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
-    "gpe_tune_set", "gpe_tune_get", "gpe_autotune",
+    "gpe_tune_set", "gpe_tune_get", "gpe_autotune", "gpe_probe_bandwidth",
 ]
 
 
@@ -97,6 +97,7 @@ def lib():
         "gpe_tune_set": ([vp, i32, i32, i32], ctypes.c_int),
         "gpe_tune_get": ([vp, i32] + [ctypes.POINTER(i32)] * 4, ctypes.c_int),
         "gpe_autotune": ([vp, i32], ctypes.c_int),
+        "gpe_probe_bandwidth": ([i64, i32, dp, dp, dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -104,6 +105,13 @@ def lib():
         f.restype = res
     _lib = L
     return L
+
+
+def probe_bandwidth(nbytes=1 << 31, reps=20):
+    """Measured HBM read / write / copy GB/s of plain float4 kernels."""
+    r, w, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    check(lib().gpe_probe_bandwidth(int(nbytes), int(reps), ctypes.byref(r), ctypes.byref(w), ctypes.byref(c)))
+    return {"read": r.value, "write": w.value, "copy": c.value}
 
 
 def check(rc):

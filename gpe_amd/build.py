@@ -41,19 +41,28 @@ def _compile(unit):
     return out
 
 
-def build(verbose=True):
+def build(verbose=True, extra_flags=(), lib_path=None, obj_dir=None):
+    """Compile every unit and link the library.  extra_flags/lib_path/obj_dir
+    build an alternative library (e.g. -DGPE_NT_STORE) beside the default."""
+    global OBJ
+    if obj_dir:
+        OBJ = obj_dir
     os.makedirs(OBJ, exist_ok=True)
+    if extra_flags:
+        global UNITS
+        UNITS = [(a, b, c + list(extra_flags)) for a, b, c in UNITS]
     with ThreadPoolExecutor(max_workers=min(len(UNITS), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(_compile, UNITS))
-    tmp = LIB + ".tmp"
+    out = lib_path or LIB
+    tmp = out + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -565,6 +565,7 @@ int gpe_ctx_create(const gpe_params *params, gpe_ctx **out) {
     A.eqn = params->eqn;
     A.use_diss = params->diss_amp != 0.0;
     A.use_urhs = params->urhs != 0.0;
+    A.scale_real = params->real_time ? 1 : 0;
     A.xl = params->xl;
     A.yl = params->yl;
     A.zl = params->zl;
@@ -899,6 +900,17 @@ int gpe_autotune(gpe_ctx *c, int32_t reps) {
   hipEventDestroy(e1);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
+  return GPE_OK;
+}
+
+int gpe_probe_bandwidth(int64_t bytes, int32_t reps, double *read_gbs, double *write_gbs, double *copy_gbs) {
+  if (bytes < (1 << 20) || reps < 1) return fail(GPE_EINVAL, "bytes >= 1 MiB and reps >= 1 required");
+  double g[3] = {0, 0, 0};
+  hipError_t e = probe_bandwidth((size_t)bytes, reps, g);
+  if (e != hipSuccess) return fail(GPE_EHIP, "bandwidth probe: %s", hipGetErrorString(e));
+  if (read_gbs) *read_gbs = g[0];
+  if (write_gbs) *write_gbs = g[1];
+  if (copy_gbs) *copy_gbs = g[2];
   return GPE_OK;
 }
 

@@ -65,4 +65,59 @@ hipError_t launch_renorm_f32(void *psi, void *lo, size_t n, double a, double b, 
   return hipGetLastError();
 }
 
+// HBM bandwidth probes: read, write and copy of float4 arrays, grid-stride.
+static __global__ void __launch_bounds__(256) bw_read(const float4 *__restrict__ a, size_t n, float *sink) {
+  float acc = 0.f;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    float4 v = a[i];
+    acc += (v.x + v.y) + (v.z + v.w);
+  }
+  if (acc == 1.2345e-30f) sink[0] = acc;  // keeps the loads live
+}
+static __global__ void __launch_bounds__(256) bw_write(float4 *__restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = make_float4(0.f, 1.f, 2.f, 3.f);
+}
+static __global__ void __launch_bounds__(256) bw_copy(const float4 *__restrict__ a, float4 *__restrict__ b,
+                                                      size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+hipError_t probe_bandwidth(size_t bytes, int reps, double *gbs) {
+  const size_t n = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  float *sink = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void **)&a, n * 16);
+  if (e == hipSuccess) e = hipMalloc((void **)&b, n * 16);
+  if (e == hipSuccess) e = hipMalloc((void **)&sink, 64);
+  if (e == hipSuccess) e = hipMemset(a, 0, n * 16);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  const int grid = 256 * 32;
+  for (int which = 0; which < 3 && e == hipSuccess; ++which) {
+    for (int r = -1; r < reps && e == hipSuccess; ++r) {
+      if (r == 0) hipEventRecord(e0, 0);
+      if (which == 0) hipLaunchKernelGGL(bw_read, dim3(grid), dim3(256), 0, 0, a, n, sink);
+      if (which == 1) hipLaunchKernelGGL(bw_write, dim3(grid), dim3(256), 0, 0, b, n);
+      if (which == 2) hipLaunchKernelGGL(bw_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
+      e = hipGetLastError();
+    }
+    if (e != hipSuccess) break;
+    hipEventRecord(e1, 0);
+    e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    const double moved = (double)n * 16 * (which == 2 ? 2.0 : 1.0) * reps;
+    gbs[which] = moved / (ms / 1e3) / 1e9;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  if (sink) hipFree(sink);
+  return e;
+}
+
 }  // namespace gpe

@@ -34,6 +34,9 @@ hipError_t launch_norm_f32(const void *psi, const void *lo, int nx, int ny, int 
 hipError_t launch_renorm_f64(void *psi, size_t n, double a, DivC b, hipStream_t s);
 hipError_t launch_renorm_f32(void *psi, void *lo, size_t n, double a, double b, hipStream_t s);
 
+// HBM read / write / copy bandwidth (GB/s) of float4 arrays of `bytes`.
+hipError_t probe_bandwidth(size_t bytes, int reps, double *gbs);
+
 // per-variant entry points (one definition per translation unit)
 #define GPE_DECL_VARIANT(P, V) \
   hipError_t launch_stage_##P##_v##V(int kind, int radius, bool comp, const StageArgs &A, hipStream_t s);

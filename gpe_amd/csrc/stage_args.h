@@ -45,7 +45,7 @@ struct StageArgs {
   int write_scale;  // K_CK_1 on the first attempt of a step
   int use_diss;     // diss_amp != 0
   int use_urhs;     // urhs != 0
-  int pad0;
+  int scale_real;   // rk45 error scale stored as one real word (real time)
   // coordinates and coefficients (ic.f90:32-69, 713-731)
   double xl, yl, zl, dx, dy, dz;
   double g, mu, ox2, oy2, oz2, urhs, diss_amp, a_diss, b_diss, c_diss;

@@ -70,10 +70,17 @@ __device__ __forceinline__ cx<T> ldc(const void *p, size_t i) {
 }
 template <typename T>
 __device__ __forceinline__ void stc(void *p, size_t i, cx<T> a) {
+#ifdef GPE_NT_STORE
+  // streaming outputs are not re-read in the sweep: keep them out of L2
+  typedef T nv2 __attribute__((ext_vector_type(2)));
+  nv2 v = {a.re, a.im};
+  __builtin_nontemporal_store(v, static_cast<nv2 *>(p) + i);
+#else
   vec2_t<T> v;
   v.x = a.re;
   v.y = a.im;
   static_cast<vec2_t<T> *>(p)[i] = v;
+#endif
 }
 
 // complex / real constant
@@ -124,15 +131,34 @@ __device__ __forceinline__ double nweight(int i, int n, double w1, double w2, do
 // ---------------------------------------------------------------------------
 // Laplacian in the reference's evaluation order (derivs.f90:164-181,
 // 209-224, 251-266; variables.f90:68).
+//
+// FP32 forms the same sums from differences with the centre first:
+// (f1 - f0) + (f-1 - f0) and 16[(f1 - f0) + (f-1 - f0)] - [(f2 - f0) + (f-2 - f0)].
+// For a smooth field neighbouring values are within a factor 2, so each
+// difference is exact (Sterbenz), and the sum no longer cancels ~64|f|
+// down to ~theta^2 |f|: the FP32 Laplacian of a well-resolved field keeps
+// ~1e-6 relative accuracy instead of ~1e-3, which keeps the rk45 error
+// estimate (itself a ~1e-6 fraction of the stage values) above rounding
+// noise.
 template <typename T, int R>
 __device__ __forceinline__ cx<T> d2(cx<T> m2, cx<T> m1, cx<T> c, cx<T> p1, cx<T> p2, const DivC &d,
                                     double inv) {
-  if constexpr (R == 1) {
-    cx<T> num = (p1 - rmul((T)2.0, c)) + m1;
-    return rdiv(num, d, inv);
+  if constexpr (std::is_same<T, double>::value) {
+    if constexpr (R == 1) {
+      cx<T> num = (p1 - rmul((T)2.0, c)) + m1;
+      return rdiv(num, d, inv);
+    } else {
+      cx<T> num = (((mk(-p2.re, -p2.im) + rmul((T)16.0, p1)) - rmul((T)30.0, c)) + rmul((T)16.0, m1)) - m2;
+      return rdiv(num, d, inv);
+    }
   } else {
-    cx<T> num = (((mk(-p2.re, -p2.im) + rmul((T)16.0, p1)) - rmul((T)30.0, c)) + rmul((T)16.0, m1)) - m2;
-    return rdiv(num, d, inv);
+    if constexpr (R == 1) {
+      return rdiv((p1 - c) + (m1 - c), d, inv);
+    } else {
+      cx<T> near = (p1 - c) + (m1 - c);
+      cx<T> far = (p2 - c) + (m2 - c);
+      return rdiv(rmul((T)16.0, near) - far, d, inv);
+    }
   }
 }
 
@@ -373,7 +399,13 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
   };
   auto load_out = [&](size_t s, cx<T> *o) {
     int i = 0;
-    if constexpr (KIND == K_CK_6) o[i++] = ldc<T>(A.scale_in, s);
+    if constexpr (KIND == K_CK_6) {
+      // real time: the scale is real (its imaginary part is 0*|f|), stored as one word
+      if (A.scale_real)
+        o[i++] = mk(static_cast<const T *>(A.scale_in)[s], (T)0);
+      else
+        o[i++] = ldc<T>(A.scale_in, s);
+    }
     if constexpr (COMP && KI::final_) o[i++] = ldc<T>(A.lo, s);
     (void)i;
   };
@@ -513,7 +545,10 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
                 // psi_scal = abs(in) + dt*abs(f) + 1e-30 (solve.f90:176-178)
                 T af = cabs_(f), au = cabs_(c0);
                 cx<T> sc = mk((au + (T)A.sdtr * af) + (T)1e-30, (T)A.sdti * af);
-                stc<T>(A.scale_out, s, sc);
+                if (A.scale_real)
+                  static_cast<T *>(A.scale_out)[s] = sc.re;
+                else
+                  stc<T>(A.scale_out, s, sc);
               }
             } else if constexpr (KIND < K_CK_6) {
               stc<T>(A.kout, s, kk);

@@ -144,6 +144,13 @@ int gpe_prof_enable(gpe_ctx *ctx, int32_t enable);
 int gpe_prof_query(gpe_ctx *ctx, int32_t kind, int64_t *launches, double *total_ms,
                    double *bytes_per_launch, char *name);
 
+/* Measured HBM ceiling of this GPU: read, write and copy bandwidth (GB/s,
+ * counting bytes read + written) of plain float4 kernels over arrays of
+ * `bytes`.  No reference counterpart: the reference point the roofline
+ * fractions are read against, beside the 8 TB/s datasheet peak. */
+int gpe_probe_bandwidth(int64_t bytes, int32_t reps, double *read_gbs, double *write_gbs,
+                        double *copy_gbs);
+
 #ifdef __cplusplus
 }
 #endif
