@@ -86,14 +86,28 @@ def run_path(precision, psi, steps, warmup, tune=True):
     }
 
 
-def roofline_of(res):
+def measured_traffic():
+    """HBM bytes per launch per kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on copy kernels)."""
+    path = os.path.join(ROOT, "profiles", "rk45_512cube_pmc_traffic.md")
+    if not os.path.exists(path):
+        return {}
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return json.loads(line.split(":", 1)[1])
+    return {}
+
+
+def roofline_of(res, precision):
     # dominant kernel = largest total event time
     kind, (n, ms, by) = max(res["prof"].items(), key=lambda kv: kv[1][1])
     avg_s = ms / n / 1e3
     achieved = by / avg_s / 1e9
+    traffic = measured_traffic().get("%s/%s" % (precision, kind))
     return {"bound": "hbm", "kernel": kind, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "avg_launch_ms": round(avg_s * 1e3, 4), "bytes_per_launch": by}
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "avg_launch_ms": round(avg_s * 1e3, 4), "bytes_per_launch": by,
+            "traffic_gbs": round(traffic / avg_s / 1e9, 1) if traffic else None}
 
 
 def step_roofline(res, bytes_per_site_attempt):
@@ -165,10 +179,10 @@ def main():
                                "FP32 storage + compensated psi update (FP64 beside)",
                    "grid": [512, 512, 512], "scheme": "rk45", "order": 4, "bcs": "periodic",
                    "seed": a.seed},
-        "roofline": roofline_of(r32),
+        "roofline": roofline_of(r32, "fp32c"),
         "cpu_baseline": cpu,
         "fp64": {"value": round(r64["su_per_s"], 1), "ms_per_step": round(r64["ms_per_step"], 4),
-                 "roofline": roofline_of(r64)},
+                 "roofline": roofline_of(r64, "fp64")},
         "fp32c_over_fp64": round(r32["su_per_s"] / r64["su_per_s"], 3),
         "hbm_probe_gbs": {k: round(v, 1) for k, v in probe.items()},
         "sim_time": {"fp32c": round(r32["t_sim"], 6), "fp64": round(r64["t_sim"], 6)},
