@@ -134,6 +134,71 @@ def cpu_baseline(psi):
                       % (1 + st.n_rejects, el)}
 
 
+def batch_leg(seed, steps=None, log_=log):
+    """64 uneven simulations (N in {32..128}, RK4, 500 steps, FP32C) in one
+    batch, against the same simulations run one after another on the plain
+    FP64 single-simulation path.  Fields are resident before each timed
+    region; the batch's launch configuration is autotuned first (commits no
+    step)."""
+    from gpe_amd.batch import Batch
+    from gpe_amd.sim import Simulation
+    from gpe_amd.workloads import batch_mix
+    mix = batch_mix(seed)
+    if steps:
+        for m in mix:
+            m.steps = steps
+    psis = [m.initial_state() for m in mix]
+    sites = sum(m.cfg.nx * m.cfg.ny * m.cfg.nz for m in mix)
+    su = float(sum(m.cfg.nx * m.cfg.ny * m.cfg.nz * m.steps for m in mix))
+    with Batch() as b:
+        ids = [b.add(m.cfg, p, m.steps) for m, p in zip(mix, psis)]
+        b.autotune(reps=2)
+        b.run(0)  # work tables and graph capture, outside the timed region
+        b.sync()
+        t0 = time.perf_counter()
+        b.run(mix[0].steps)
+        b.sync()
+        el_total = time.perf_counter() - t0
+        done = dict(b.poll())
+        assert len(done) == len(ids) and all(v == 1 for v in done.values()), done
+        got = [b.download(i) for i in ids]
+        tiles = {k: b.tune_get("fp32c", 2, "rk4", k) for k in ("rk4_1", "rk4_2", "rk4_3", "rk4_4")}
+    log_("batch fp32c: %d solves in %.3f s" % (len(ids), el_total))
+    # sequential plain FP64, one simulation after another
+    seq = 0.0
+    errs = []
+    for m, p, g in zip(mix, psis, got):
+        c = m.cfg.with_(precision="fp64")
+        with Simulation(c, p) as sim:
+            sim.advance(1)
+            sim.set_state(c.dt)
+            sim.upload(p)
+            t0 = time.perf_counter()
+            sim.advance(m.steps)
+            seq += time.perf_counter() - t0
+            ref = sim.download()
+        errs.append(float(np.linalg.norm(g - ref) / np.linalg.norm(ref)))
+    log_("sequential fp64: %.3f s" % seq)
+    return {"instances": len(mix), "sites": sites, "steps": mix[0].steps,
+            "solves_per_s": round(len(mix) / el_total, 3),
+            "seq_fp64_solves_per_s": round(len(mix) / seq, 3),
+            "speedup_vs_seq_fp64": round(seq / el_total, 3),
+            "site_updates_per_s": round(su / el_total, 1),
+            "achieved_gbs": round(su * 120.0 / el_total / 1e9, 1),
+            "psi_rel_l2_vs_fp64_max": max(errs), "tiles": tiles}
+
+
+def serve_leg(rate, duration, seed, log_=log):
+    """Poisson request stream into the continuous-batching server."""
+    from gpe_amd.serve import Server
+    from gpe_amd.workloads import request_stream
+    reqs = request_stream(rate, duration, seed)
+    with Server(max_sites=64e6) as s:
+        m = s.serve(reqs, duration=duration, drain=False, log=lambda *x: log_("serve", *x))
+    m.update({"rate": rate, "duration_s": duration, "requests": len(reqs)})
+    return m
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +206,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-simulation batch leg")
+    ap.add_argument("--serve-seconds", type=float, default=30.0,
+                    help="length of the Poisson serving leg (0 to skip)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -158,6 +226,10 @@ def main():
     r64 = run_path("fp64", psi, a.steps, a.warmup)
     log("fp64  %.3e su/s, %.3f ms/step" % (r64["su_per_s"], r64["ms_per_step"]))
 
+    del psi
+    bat = None if a.no_batch else batch_leg(a.seed)
+    srv = serve_leg(40.0, a.serve_seconds, a.seed) if a.serve_seconds > 0 else None
+    psi = ic.random_phase(workload("fp64"), a.seed)
     cpu = None if a.no_cpu else cpu_baseline(psi)
     if cpu:
         log("cpu baseline %.3e su/s on %d threads" % (cpu["value"], cpu["cores"]))
@@ -191,6 +263,8 @@ def main():
         "step_gbs": {"fp32c": step_roofline(r32, 232.0), "fp64": step_roofline(r64, 432.0)},
         "kernels": {"fp32c": {k: [v[0], round(v[1], 3)] for k, v in r32["prof"].items()},
                     "fp64": {k: [v[0], round(v[1], 3)] for k, v in r64["prof"].items()}},
+        "batch": bat,
+        "serve": srv,
     }
     print(json.dumps(out), flush=True)
 

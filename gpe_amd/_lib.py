@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("GPE_LIBRARY") or os.path.join(HERE, "libgpehip.so")
 
 GPE_OK = 0
+GPE_EFULL = -7
 GPE_EULER, GPE_RK2, GPE_RK4, GPE_RK45 = 1, 2, 4, 45
 GPE_FP64, GPE_FP32, GPE_FP32C = 0, 1, 2
 GPE_HOST_C128, GPE_HOST_C64 = 0, 1
@@ -29,6 +30,10 @@ EXPORTS = [
     "gpe_sim_advance", "gpe_sim_norm_history", "gpe_sim_norm", "gpe_sim_energy",
     "gpe_sim_mass", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
     "gpe_tune_set", "gpe_tune_get", "gpe_autotune", "gpe_probe_bandwidth",
+    "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_footprint", "gpe_batch_add",
+    "gpe_batch_remove", "gpe_batch_run", "gpe_batch_sync", "gpe_batch_poll", "gpe_batch_query",
+    "gpe_batch_download", "gpe_batch_stats", "gpe_batch_use_graph", "gpe_batch_tune_set",
+    "gpe_batch_tune_get", "gpe_batch_autotune",
 ]
 
 
@@ -98,6 +103,22 @@ def lib():
         "gpe_tune_get": ([vp, i32] + [ctypes.POINTER(i32)] * 4, ctypes.c_int),
         "gpe_autotune": ([vp, i32], ctypes.c_int),
         "gpe_probe_bandwidth": ([i64, i32, dp, dp, dp], ctypes.c_int),
+        "gpe_batch_create": ([i64, ctypes.POINTER(vp)], ctypes.c_int),
+        "gpe_batch_destroy": ([vp], None),
+        "gpe_batch_footprint": ([P, ctypes.POINTER(i64)], ctypes.c_int),
+        "gpe_batch_add": ([vp, P, vp, i32, S, i64, ctypes.POINTER(i32)], ctypes.c_int),
+        "gpe_batch_remove": ([vp, i32], ctypes.c_int),
+        "gpe_batch_run": ([vp, i32], ctypes.c_int),
+        "gpe_batch_sync": ([vp], ctypes.c_int),
+        "gpe_batch_poll": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i32), i32, ctypes.POINTER(i32)], ctypes.c_int),
+        "gpe_batch_query": ([vp, i32, S, ctypes.POINTER(i64), ctypes.POINTER(i32), dp], ctypes.c_int),
+        "gpe_batch_download": ([vp, i32, vp, i32], ctypes.c_int),
+        "gpe_batch_stats": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i64),
+                             ctypes.POINTER(i64)], ctypes.c_int),
+        "gpe_batch_use_graph": ([vp, i32], ctypes.c_int),
+        "gpe_batch_tune_set": ([vp, i32, i32, i32, i32, i32, i32], ctypes.c_int),
+        "gpe_batch_tune_get": ([vp, i32, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)], ctypes.c_int),
+        "gpe_batch_autotune": ([vp, i32], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -15,23 +15,13 @@
 #include <vector>
 
 #include "../../include/gpe_hip.h"
+#include "host_common.h"
 #include "launch.h"
 
 using namespace gpe;
 
+
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const char *fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return code;
-}
 
 #define HIPCHK(call)                                                                        \
   do {                                                                                      \
@@ -39,15 +29,10 @@ int fail(int code, const char *fmt, ...) {
     if (e_ != hipSuccess) return fail(GPE_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
-// Division by a real constant in the reference's form (see stage_args.h).
-DivC make_divc(double s) {
-  DivC d{};
-  int e = ilogb(s);
-  d.e = e;
-  d.cs = ldexp(s, -e);
-  d.den = d.cs * d.cs + 0.0 * 0.0;
-  return d;
-}
+}  // namespace
+
+
+namespace {
 
 const char *kind_name(int k) {
   static const char *names[] = {"rhs",   "euler", "rk2_1", "rk2_2", "rk4_1", "rk4_2", "rk4_3", "rk4_4",
@@ -82,6 +67,7 @@ int kind_passes(int k) {
 }
 
 }  // namespace
+
 
 struct gpe_ctx {
   gpe_params P{};
@@ -414,84 +400,14 @@ int alloc_field(gpe_ctx *c, void **p) {
   return GPE_OK;
 }
 
-// Host <-> device conversions for uploads/downloads.
 int to_device(gpe_ctx *c, const void *src, int host_type, size_t src_pitch_planes, size_t row_off) {
-  // src_pitch_planes: elements between z-planes in the host array; row_off:
-  // element offset of (0, 0, 0) of the interior.
-  const size_t plane = (size_t)c->P.nx * c->P.ny;
-  const bool contiguous = src_pitch_planes == plane && row_off == 0;
-  if (c->prec == GPE_FP64 && host_type == GPE_HOST_C128) {
-    if (contiguous) {
-      HIPCHK(hipMemcpy(c->psi, src, c->n * 16, hipMemcpyHostToDevice));
-    } else {
-      HIPCHK(hipMemcpy2D(c->psi, plane * 16, (const char *)src + row_off * 16, src_pitch_planes * 16, plane * 16,
-                         c->P.nz, hipMemcpyHostToDevice));
-    }
-    return GPE_OK;
-  }
-  if (c->prec == GPE_FP32 && host_type == GPE_HOST_C64) {
-    if (contiguous) {
-      HIPCHK(hipMemcpy(c->psi, src, c->n * 8, hipMemcpyHostToDevice));
-    } else {
-      HIPCHK(hipMemcpy2D(c->psi, plane * 8, (const char *)src + row_off * 8, src_pitch_planes * 8, plane * 8,
-                         c->P.nz, hipMemcpyHostToDevice));
-    }
-    return GPE_OK;
-  }
-  if (c->prec == GPE_FP64) return fail(GPE_EINVAL, "FP64 context takes complex128 host data");
-  // FP32 / FP32C from complex128: split into hi (and lo) on the host.
-  if (host_type != GPE_HOST_C128 && c->comp) {
-    // complex64 into a compensated context: lo = 0
-    std::vector<float> hi(2 * plane);
-    for (int k = 0; k < c->P.nz; ++k) {
-      const float *s = (const float *)src + 2 * (row_off + (size_t)k * src_pitch_planes);
-      HIPCHK(hipMemcpy((char *)c->psi + (size_t)k * plane * 8, s, plane * 8, hipMemcpyHostToDevice));
-    }
-    HIPCHK(hipMemset(c->lo, 0, c->n * 8));
-    return GPE_OK;
-  }
-  if (host_type != GPE_HOST_C128) return fail(GPE_EINVAL, "unsupported host type");
-  std::vector<float> hi(2 * plane), lo(2 * plane);
-  for (int k = 0; k < c->P.nz; ++k) {
-    const double *s = (const double *)src + 2 * (row_off + (size_t)k * src_pitch_planes);
-    for (size_t q = 0; q < 2 * plane; ++q) {
-      float h = (float)s[q];
-      hi[q] = h;
-      lo[q] = (float)(s[q] - (double)h);
-    }
-    HIPCHK(hipMemcpy((char *)c->psi + (size_t)k * plane * 8, hi.data(), plane * 8, hipMemcpyHostToDevice));
-    if (c->comp)
-      HIPCHK(hipMemcpy((char *)c->lo + (size_t)k * plane * 8, lo.data(), plane * 8, hipMemcpyHostToDevice));
-  }
-  return GPE_OK;
+  return field_upload(c->prec, (size_t)c->P.nx * c->P.ny, c->P.nz, src, host_type, src_pitch_planes, row_off,
+                      c->psi, c->lo);
 }
 
 int from_device(gpe_ctx *c, void *dst, int host_type) {
-  const size_t plane = (size_t)c->P.nx * c->P.ny;
-  HIPCHK(hipStreamSynchronize(c->stream));
-  if (c->prec == GPE_FP64) {
-    if (host_type != GPE_HOST_C128) return fail(GPE_EINVAL, "FP64 context returns complex128 host data");
-    HIPCHK(hipMemcpy(dst, c->psi, c->n * 16, hipMemcpyDeviceToHost));
-    return GPE_OK;
-  }
-  if (host_type == GPE_HOST_C64 && !c->comp) {
-    HIPCHK(hipMemcpy(dst, c->psi, c->n * 8, hipMemcpyDeviceToHost));
-    return GPE_OK;
-  }
-  std::vector<float> hi(2 * plane), lo(2 * plane, 0.f);
-  for (int k = 0; k < c->P.nz; ++k) {
-    HIPCHK(hipMemcpy(hi.data(), (char *)c->psi + (size_t)k * plane * 8, plane * 8, hipMemcpyDeviceToHost));
-    if (c->comp)
-      HIPCHK(hipMemcpy(lo.data(), (char *)c->lo + (size_t)k * plane * 8, plane * 8, hipMemcpyDeviceToHost));
-    if (host_type == GPE_HOST_C128) {
-      double *d = (double *)dst + 2 * (size_t)k * plane;
-      for (size_t q = 0; q < 2 * plane; ++q) d[q] = (double)hi[q] + (double)lo[q];
-    } else {
-      float *d = (float *)dst + 2 * (size_t)k * plane;
-      for (size_t q = 0; q < 2 * plane; ++q) d[q] = (float)((double)hi[q] + (double)lo[q]);
-    }
-  }
-  return GPE_OK;
+  GPE_HIPCHK(hipStreamSynchronize(c->stream));
+  return field_download(c->prec, (size_t)c->P.nx * c->P.ny, c->P.nz, c->psi, c->lo, dst, host_type);
 }
 
 }  // namespace
@@ -556,78 +472,7 @@ int gpe_ctx_create(const gpe_params *params, gpe_ctx **out) {
       goto bad;
     }
   }
-  {
-    StageArgs &A = c->base;
-    A.nx = params->nx;
-    A.ny = params->ny;
-    A.nz = params->nz;
-    A.periodic = params->bcs == 1;
-    A.eqn = params->eqn;
-    A.use_diss = params->diss_amp != 0.0;
-    A.use_urhs = params->urhs != 0.0;
-    A.scale_real = params->real_time ? 1 : 0;
-    A.xl = params->xl;
-    A.yl = params->yl;
-    A.zl = params->zl;
-    A.dx = params->dx;
-    A.dy = params->dy;
-    A.dz = params->dz;
-    A.g = params->g;
-    A.mu = params->mu;
-    A.ox2 = params->omx * params->omx;
-    A.oy2 = params->omy * params->omy;
-    A.oz2 = params->omz * params->omz;
-    A.urhs = params->urhs;
-    A.diss_amp = params->diss_amp;
-    // a = x(nx-3) etc. (solve.f90:397-399)
-    A.a_diss = params->xl + (double)(params->nx - 3) * params->dx;
-    A.b_diss = params->yl + (double)(params->ny - 3) * params->dy;
-    A.c_diss = params->zl + (double)(params->nz - 3) * params->dz;
-    const double dx2 = params->dx * params->dx, dy2 = params->dy * params->dy, dz2 = params->dz * params->dz;
-    const double sxx = c->R == 1 ? dx2 : 12.0 * dx2;
-    const double syy = c->R == 1 ? dy2 : 12.0 * dy2;
-    const double szz = c->R == 1 ? dz2 : 12.0 * dz2;
-    const double sx = c->R == 1 ? 2.0 * params->dx : 12.0 * params->dx;
-    if (c->prec == GPE_FP32) {
-      // the reference built with pr = single forms these in single precision
-      const float fdx = (float)params->dx, fdy = (float)params->dy, fdz = (float)params->dz;
-      const float fx2 = fdx * fdx, fy2 = fdy * fdy, fz2 = fdz * fdz;
-      A.inv_xx = 1.0 / (double)(c->R == 1 ? fx2 : 12.0f * fx2);
-      A.inv_yy = 1.0 / (double)(c->R == 1 ? fy2 : 12.0f * fy2);
-      A.inv_zz = 1.0 / (double)(c->R == 1 ? fz2 : 12.0f * fz2);
-      A.inv_x = 1.0 / (double)(c->R == 1 ? 2.0f * fdx : 12.0f * fdx);
-    } else {
-      A.inv_xx = 1.0 / sxx;
-      A.inv_yy = 1.0 / syy;
-      A.inv_zz = 1.0 / szz;
-      A.inv_x = 1.0 / sx;
-    }
-    A.d_xx = make_divc(sxx);
-    A.d_yy = make_divc(syy);
-    A.d_zz = make_divc(szz);
-    A.d_x = make_divc(sx);
-    A.d6 = make_divc(6.0);
-    A.d3 = make_divc(3.0);
-    // Cash-Karp weights (solve.f90:270-281), formed in double as the
-    // reference forms its real(pr) parameters
-    const double c1 = 37.0 / 378.0, c2 = 0.0, c3 = 250.0 / 621.0, c4 = 125.0 / 594.0, c5 = 0.0,
-                 c6 = 512.0 / 1771.0;
-    A.c[1] = c1;
-    A.c[2] = c2;
-    A.c[3] = c3;
-    A.c[4] = c4;
-    A.c[5] = c5;
-    A.c[6] = c6;
-    A.dc[1] = c1 - 2825.0 / 27648.0;
-    A.dc[2] = c2 - 0.0;
-    A.dc[3] = c3 - 18575.0 / 48384.0;
-    A.dc[4] = c4 - 13525.0 / 55296.0;
-    A.dc[5] = c5 - 277.0 / 14336.0;
-    A.dc[6] = c6 - 0.25;
-    A.w1 = 3.0 / 8.0;
-    A.w2 = 7.0 / 6.0;
-    A.w3 = 23.0 / 24.0;
-  }
+  fill_base(params, c->prec, c->base);
   *out = c;
   return GPE_OK;
 bad:

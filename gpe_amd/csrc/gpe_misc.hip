@@ -1,5 +1,6 @@
 // gpe_misc.hip -- reductions and pointwise kernels, and the dispatch over
 // the stage-kernel variants.  Built with -ffp-contract=off.
+#include "batch_ctl.hpp"
 #include "launch.h"
 #include "pointwise.hpp"
 
@@ -25,6 +26,35 @@ hipError_t launch_stage(int prec_bytes, int vid, int kind, int radius, bool comp
     }
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_stage_batched(int prec_bytes, int vid, int kind, int radius, bool comp, const BatchLaunch &B,
+                                hipStream_t s) {
+  if (prec_bytes == 16) {
+    switch (vid) {
+      case 0: return launch_batched_f64_v0(kind, radius, comp, B, s);
+      case 1: return launch_batched_f64_v1(kind, radius, comp, B, s);
+      case 2: return launch_batched_f64_v2(kind, radius, comp, B, s);
+      case 3: return launch_batched_f64_v3(kind, radius, comp, B, s);
+      case 4: return launch_batched_f64_v4(kind, radius, comp, B, s);
+    }
+  } else {
+    switch (vid) {
+      case 0: return launch_batched_f32_v0(kind, radius, comp, B, s);
+      case 1: return launch_batched_f32_v1(kind, radius, comp, B, s);
+      case 2: return launch_batched_f32_v2(kind, radius, comp, B, s);
+      case 3: return launch_batched_f32_v3(kind, radius, comp, B, s);
+      case 4: return launch_batched_f32_v4(kind, radius, comp, B, s);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_control(InstCtl *ctl, StageArgs *args, int cap, int *active, const int *range,
+                                const double *pn, const double *pe, hipStream_t s) {
+  if (cap <= 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_control, dim3(cap), dim3(64), 0, s, ctl, args, cap, active, range, pn, pe);
+  return hipGetLastError();
 }
 
 hipError_t launch_reduce(const double *pn, const double *pe, const double *pen, int n, double scale_norm,

@@ -1,0 +1,31 @@
+// host_common.h -- host helpers shared by the single-simulation and batch
+// entry points.
+#pragma once
+#include "../../include/gpe_hip.h"
+#include "stage_args.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+namespace gpe {
+// Record the message of a failed call (read back by gpe_last_error) and
+// return code.
+int fail(int code, const char *fmt, ...);
+#define GPE_HIPCHK(call)                                                                             \
+  do {                                                                                               \
+    hipError_t e_ = (call);                                                                          \
+    if (e_ != hipSuccess) return ::gpe::fail(GPE_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+DivC make_divc(double s);
+// Geometry, coefficients and constant divisors of one simulation (no field
+// pointers, no dt).
+void fill_base(const gpe_params *params, int prec, StageArgs &A);
+// Field conversions between host arrays and device storage (hi, and lo for
+// the compensated form).
+int field_upload(int prec, size_t plane, int nz, const void *src, int host_type, size_t src_pitch_planes,
+                 size_t row_off, void *hi, void *lo);
+int field_download(int prec, size_t plane, int nz, const void *hi, const void *lo, void *dst, int host_type);
+extern thread_local std::string g_err;
+}  // namespace gpe

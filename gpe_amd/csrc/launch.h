@@ -21,6 +21,22 @@ constexpr Tile kTiles[N_TILE_VARIANTS] = {{4, 4}, {8, 2}, {16, 1}, {4, 2}, {8, 1
 hipError_t launch_stage(int prec_bytes, int vid, int kind, int radius, bool comp, const StageArgs &A,
                         hipStream_t s);
 
+// Batched launch over a work table (see stage_kernel_batched).
+struct BatchLaunch {
+  const StageArgs *insts;  // this kind's StageArgs of every instance slot
+  const WorkItem *work;
+  int nwork;
+  const int *active;
+  double *pn, *pe;  // partial sums of the final stage (null otherwise)
+  int grid;
+};
+hipError_t launch_stage_batched(int prec_bytes, int vid, int kind, int radius, bool comp, const BatchLaunch &B,
+                                hipStream_t s);
+
+// Per-instance step control after the final stage of a batched iteration.
+hipError_t launch_batch_control(InstCtl *ctl, StageArgs *args, int cap, int *active, const int *range,
+                                const double *pn, const double *pe, hipStream_t s);
+
 // Reduce per-block partials to out[0..2] (see reduce_partials).
 hipError_t launch_reduce(const double *pn, const double *pe, const double *pen, int n, double scale_norm,
                          double eps, double scale_energy, double *out, double *hist, hipStream_t s);
@@ -51,5 +67,18 @@ GPE_DECL_VARIANT(f32, 2)
 GPE_DECL_VARIANT(f32, 3)
 GPE_DECL_VARIANT(f32, 4)
 #undef GPE_DECL_VARIANT
+#define GPE_DECL_BVARIANT(P, V)                                                                            \
+  hipError_t launch_batched_##P##_v##V(int kind, int radius, bool comp, const BatchLaunch &B, hipStream_t s);
+GPE_DECL_BVARIANT(f64, 0)
+GPE_DECL_BVARIANT(f64, 1)
+GPE_DECL_BVARIANT(f64, 2)
+GPE_DECL_BVARIANT(f64, 3)
+GPE_DECL_BVARIANT(f64, 4)
+GPE_DECL_BVARIANT(f32, 0)
+GPE_DECL_BVARIANT(f32, 1)
+GPE_DECL_BVARIANT(f32, 2)
+GPE_DECL_BVARIANT(f32, 3)
+GPE_DECL_BVARIANT(f32, 4)
+#undef GPE_DECL_BVARIANT
 
 }  // namespace gpe

@@ -79,4 +79,30 @@ struct StageArgs {
   double *part_energy;
 };
 
+// A tile of one instance in a batched sweep.
+struct WorkItem {
+  int inst;    // instance slot (index into the kind's StageArgs table)
+  int tx, ty;  // 64 x TY column tile
+  int z0, z1;  // planes z0 .. z1-1
+  int slot;    // partial-sum slot: contiguous per instance, in tile order
+};
+
+// Per-instance step state of a batch (device memory; see batch_ctl.hpp).
+struct InstCtl {
+  int active;       // 1 while steps remain
+  int status;       // 0 running, 1 done, -1 NaN in the error norm, -2 too many rejections
+  int scheme;       // 1, 2, 4 or 45
+  int real_time;
+  int cur;          // which of buf[0], buf[1] holds the state
+  int first_attempt;
+  int n_rejects;    // rejections in the current rk45 step
+  int last_rejects; // rejections of the last accepted step
+  long long steps_done, steps_target;
+  int k0, nk;       // stage kinds of the scheme: args[(k0 .. k0+nk-1) * cap + slot]
+  double t, im_t, dt_re, dt_im, sdt_re, sdt_im;
+  double errmax, norm;
+  double safety, eps, dt_inc, dt_dec, errcon, dv;
+  void *buf[2], *lob[2];
+};
+
 }  // namespace gpe

@@ -9,7 +9,8 @@
 //     (so only one field is staged, whatever the stage), into a register
 //     queue of the 2R+1 z-neighbours (R = order/2),
 //   * writes plane k (queue centre) and its x/y ghost cells into a
-//     double-buffered LDS tile,
+//     double-buffered LDS tile (the ghost cells were loaded R planes earlier,
+//     together with the neighbouring tile's load of the same rows),
 //   * evaluates the Laplacian (x and y from LDS, z from the queue), the GP
 //     right-hand side and the stage outputs for plane k,
 //   * keeps the pointwise partial sums the final combination needs in a
@@ -255,10 +256,18 @@ struct OutInfo {
 
 // Workgroup barrier that waits only for this wave's LDS traffic: the global
 // loads issued ahead for the next plane stay in flight across it.
+#ifdef GPE_PLAIN_BARRIER
+__device__ __forceinline__ void lds_barrier() { __syncthreads(); }
+#else
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+#endif
 
+// One tile (64 x TY columns, planes z0 .. z1-1) of one stage sweep.  The per-block
+// partial sums go to slot `slot` of the partial arrays pn / pe / pen (any
+// of them may be null).
 template <typename T, int R, int NW, int P, int KIND, bool COMP>
-__global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
+__device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, const int ty, const int z0,
+                                           const int z1, const int slot, double *pn, double *pe, double *pen) {
   constexpr int TX = 64, TY = NW * P, LX = TX + 2 * R, LY = TY + 2 * R;
   constexpr int NQ = 2 * R + 1;
   constexpr int NT = 64 * NW;
@@ -286,21 +295,7 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
 
-  // XCD-aware tile order: the hardware deals consecutive workgroups round
-  // robin over the 8 XCDs; give each XCD a contiguous range of tiles so
-  // neighbouring tiles (which share ghost rows) run on one L2.
-  const int nb = A.nblocks;
-  const int b = blockIdx.x;
-  const int xcd = b & 7, loc = b >> 3;
-  const int q8 = nb >> 3, r8 = nb & 7;
-  const int L = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
-  const int tx = L % A.ntx;
-  const int rest = L / A.ntx;
-  const int ty = rest % A.nty;
-  const int tz = rest / A.nty;
   const int tx0 = tx * TX, ty0 = ty * TY;
-  const int z0 = tz * A.zc;
-  const int z1 = min(z0 + A.zc, A.nz);
 
   const int nx = A.nx, ny = A.ny, nz = A.nz, per = A.periodic;
   const size_t plane = (size_t)nx * (size_t)ny;
@@ -412,8 +407,12 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
 
   cx<T> q[NQ][P];
   cx<T> F[R + 1][P][NFA];
+  // Ghost-cell stage inputs ride a FIFO of R+1 planes too: they are loaded
+  // with plane k+R, in the same iteration the neighbouring tile loads those
+  // rows as its own interior, so the second read of each row finds it in L2.
+  cx<T> G[R + 1][HPT];
 
-  // Prime the queue with planes z0-R .. z0+R-1 (and the FIFO with z0 ..).
+  // Prime the queue with planes z0-R .. z0+R-1 (and the FIFOs with z0 ..).
 #pragma unroll
   for (int m = 0; m < 2 * R; ++m) {
     const size_t zoff = (size_t)bcmap(z0 - R + m, nz, per) * plane;
@@ -426,9 +425,24 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
         if (m >= R) make_F(r, F[m - R][p]);
       }
     }
+#ifndef GPE_GHOST_LATE
+    if (m >= R) {
+#pragma unroll
+      for (int i = 0; i < HPT; ++i) {
+        if (hl[i] >= 0) {
+          Raw r;
+          load_raw(zoff + hoff[i], r, false);
+          G[m - R][i] = make_u(r);
+        } else {
+          G[m - R][i] = mk((T)0, (T)0);
+        }
+      }
+    }
+#endif
   }
 
-  // Loads for plane k: interior of k+R, ghosts of k, output-time fields of k.
+  // Loads issued for iteration k: interior and ghosts of plane k+R,
+  // output-time fields of plane k.
   Raw rn[P];
   Raw hn[HPT];
   cx<T> on[P][NOA];
@@ -439,7 +453,11 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
     for (int p = 0; p < P; ++p) load_raw(zi + roff[p], rn[p], true);
 #pragma unroll
     for (int i = 0; i < HPT; ++i)
+#ifdef GPE_GHOST_LATE
       if (hl[i] >= 0) load_raw(zk + hoff[i], hn[i], false);
+#else
+      if (hl[i] >= 0) load_raw(zi + hoff[i], hn[i], false);
+#endif
     if constexpr (NO > 0) {
 #pragma unroll
       for (int p = 0; p < P; ++p) load_out(zk + roff[p], on[p]);
@@ -457,9 +475,12 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
       q[2 * R][p] = make_u(rn[p]);
       if constexpr (NF > 0) make_F(rn[p], F[R][p]);
     }
-    cx<T> hu[HPT];
 #pragma unroll
-    for (int i = 0; i < HPT; ++i) hu[i] = hl[i] >= 0 ? make_u(hn[i]) : mk((T)0, (T)0);
+#ifdef GPE_GHOST_LATE
+    for (int i = 0; i < HPT; ++i) G[0][i] = hl[i] >= 0 ? make_u(hn[i]) : mk((T)0, (T)0);
+#else
+    for (int i = 0; i < HPT; ++i) G[R][i] = hl[i] >= 0 ? make_u(hn[i]) : mk((T)0, (T)0);
+#endif
     cx<T> oc[P][NOA];
     if constexpr (NO > 0) {
 #pragma unroll
@@ -474,8 +495,8 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
     for (int i = 0; i < HPT; ++i) {
       if (hl[i] >= 0) {
         V v;
-        v.x = hu[i].re;
-        v.y = hu[i].im;
+        v.x = G[0][i].re;
+        v.y = G[0][i].im;
         Ld[hl[i]] = v;
       }
     }
@@ -626,11 +647,15 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
       }
     }
 
-    // shift queue and FIFO
+    // shift queue and FIFOs
 #pragma unroll
     for (int m = 0; m < 2 * R; ++m)
 #pragma unroll
       for (int p = 0; p < P; ++p) q[m][p] = q[m + 1][p];
+#pragma unroll
+    for (int m = 0; m < R; ++m)
+#pragma unroll
+      for (int i = 0; i < HPT; ++i) G[m][i] = G[m + 1][i];
     if constexpr (NF > 0) {
 #pragma unroll
       for (int m = 0; m < R; ++m)
@@ -664,10 +689,48 @@ __global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
         se += red[2][w];
         mx = (red[1][w] > mx || red[1][w] != red[1][w]) ? red[1][w] : mx;
       }
-      if (A.part_norm) A.part_norm[b] = sn;
-      if (A.part_emax) A.part_emax[b] = mx;
-      if (A.part_energy) A.part_energy[b] = se;
+      if (pn) pn[slot] = sn;
+      if (pe) pe[slot] = mx;
+      if (pen) pen[slot] = se;
     }
+  }
+}
+
+// Single simulation: one workgroup per tile.
+template <typename T, int R, int NW, int P, int KIND, bool COMP>
+__global__ void __launch_bounds__(64 * NW) stage_kernel(const StageArgs A) {
+  // XCD-aware tile order: the hardware deals consecutive workgroups round
+  // robin over the 8 XCDs; give each XCD a contiguous range of tiles so
+  // neighbouring tiles (which share ghost rows) run on one L2.
+  const int nb = A.nblocks;
+  const int b = blockIdx.x;
+  const int xcd = b & 7, loc = b >> 3;
+  const int q8 = nb >> 3, r8 = nb & 7;
+  const int L = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
+  const int tx = L % A.ntx;
+  const int rest = L / A.ntx;
+  const int ty = rest % A.nty;
+  const int tz = rest / A.nty;
+  const int z0 = tz * A.zc;
+  stage_tile<T, R, NW, P, KIND, COMP>(A, tx, ty, z0, min(z0 + A.zc, A.nz), b, A.part_norm, A.part_emax,
+                                      A.part_energy);
+}
+
+// Many simulations: a persistent grid walks a table of (instance, tile)
+// work items.  Each instance's parameters (grid, coefficients, dt, field
+// pointers) live in device memory, where the per-instance step control
+// updates dt and swaps the state buffers between launches; instances that
+// have finished are skipped.  Partials go to the item's slot of the batch
+// arrays, so each instance's reduction order does not depend on the others.
+template <typename T, int R, int NW, int P, int KIND, bool COMP>
+__global__ void __launch_bounds__(64 * NW)
+    stage_kernel_batched(const StageArgs *__restrict__ insts, const WorkItem *__restrict__ work, int nwork,
+                         const int *__restrict__ active, double *pn, double *pe) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem it = work[w];
+    if (active[it.inst])
+      stage_tile<T, R, NW, P, KIND, COMP>(insts[it.inst], it.tx, it.ty, it.z0, it.z1, it.slot, pn, pe, nullptr);
+    __syncthreads();  // the next item reuses the LDS tile
   }
 }
 

@@ -1,0 +1,201 @@
+"""Continuous-batching service for GP simulations on one GPU.
+
+Requests (grid, initial state, trap, steps, scheme, precision) arrive over
+time.  The server keeps every admitted simulation's field and its RK work
+fields resident in HBM inside one `Batch`, admits queued requests first come
+first served while their device footprint fits the HBM budget (and the
+resident site count stays under a cap that keeps the persistent sweeps full
+without stretching every iteration), and runs the batch in chunks of
+iterations; between chunks it retires finished requests (result copied to
+the host, memory returned to the pool) and admits new ones, so new work
+joins the running step instead of waiting for a drain.
+
+The reference has no service mode; each request's result equals the same
+simulation run alone (see tests/test_serve.py).
+
+    python -m gpe_amd.serve --rate 40 --duration 120
+"""
+import argparse
+import json
+import sys
+import time
+from collections import deque
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from . import batch as _batch
+from .batch import Batch
+
+
+class _Ready:
+    """A computed initial state kept for a request still in the queue."""
+
+    def __init__(self, v):
+        self.v = v
+
+    def result(self):
+        return self.v
+
+    def cancel(self):
+        return False
+
+
+class Server:
+    def __init__(self, hbm_budget=0, max_sites=None, target_chunk_s=0.05, ic_workers=4, prefetch=8,
+                 deliver=None):
+        """hbm_budget: device bytes for resident simulations (0: 90% of free).
+        max_sites: cap on resident grid points (None: no cap).
+        deliver(request, psi, info): called with each finished result."""
+        self.batch = Batch(hbm_budget)
+        self.budget = self.batch.stats()["budget"]
+        self.max_sites = max_sites
+        self.target_chunk_s = target_chunk_s
+        self.pool = ThreadPoolExecutor(max_workers=ic_workers)
+        self.prefetch = prefetch
+        self.deliver = deliver
+
+    def close(self):
+        self.pool.shutdown(wait=True)
+        self.batch.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def serve(self, requests, duration=None, drain=True, clock=None, log=None):
+        """Serve `requests` (sorted by .arrival, seconds from the start).
+
+        duration: stop admitting at this time (None: admit all).  drain: keep
+        running after the last admission until every admitted request is
+        done.  clock: callable giving seconds since the start (default: wall
+        clock).  Returns the metrics dict."""
+        t0 = time.perf_counter()
+        if clock is None:
+            clock = lambda: time.perf_counter() - t0  # noqa: E731
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        queue = deque()
+        futures = {}
+        running = {}      # batch id -> (request, admit time, sites)
+        done = []         # (request, arrival, admitted, finished, status, steps)
+        resident_sites = 0
+        chunk = 4
+        busy_s = 0.0
+        su_done = 0.0
+        last_log = 0.0
+
+        def prefetch():
+            for r in list(queue)[: self.prefetch]:
+                if r.rid not in futures:
+                    futures[r.rid] = self.pool.submit(r.initial_state)
+
+        while True:
+            now = clock()
+            while pending and pending[0].arrival <= now and (duration is None or pending[0].arrival < duration):
+                queue.append(pending.popleft())
+            prefetch()
+            # admission, first come first served
+            while queue:
+                r = queue[0]
+                sites = r.cfg.nx * r.cfg.ny * r.cfg.nz
+                if self.max_sites is not None and running and resident_sites + sites > self.max_sites:
+                    break
+                if _batch.footprint(r.cfg) > self.budget:
+                    raise ValueError("request %d can never fit the HBM budget" % r.rid)
+                f = futures.pop(r.rid, None)
+                psi = f.result() if f is not None else r.initial_state()
+                iid = self.batch.add(r.cfg, psi, r.steps)
+                if iid is None:  # does not fit now: wait for memory
+                    futures[r.rid] = _Ready(psi)
+                    break
+                queue.popleft()
+                running[iid] = (r, clock(), sites)
+                resident_sites += sites
+                prefetch()
+            no_more = not pending or (duration is not None and pending[0].arrival >= duration)
+            if not running:
+                if no_more and not queue:
+                    break
+                # idle until the next arrival
+                if pending:
+                    wait = pending[0].arrival - clock()
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
+                continue
+            if duration is not None and not drain and clock() >= duration:
+                break
+            ts = time.perf_counter()
+            self.batch.run(chunk)
+            finished = self.batch.poll()
+            el = time.perf_counter() - ts
+            busy_s += el
+            # size the next chunk for ~target_chunk_s of device time
+            per_it = el / chunk
+            chunk = int(max(1, min(256, round(self.target_chunk_s / max(per_it, 1e-6)))))
+            tnow = clock()
+            for iid, status in finished:
+                r, tadm, sites = running.pop(iid)
+                q = self.batch.query(iid)
+                psi = self.batch.download(iid) if self.deliver is not None else None
+                self.batch.remove(iid)
+                resident_sites -= sites
+                su_done += float(sites) * q["steps"]
+                done.append((r, r.arrival, tadm, tnow, status, q["steps"]))
+                if self.deliver is not None:
+                    self.deliver(r, psi, q)
+            if log is not None and tnow - last_log >= 5.0:
+                last_log = tnow
+                log("t=%.1fs running %d queued %d done %d chunk %d" % (tnow, len(running), len(queue), len(done),
+                                                                      chunk))
+        end = clock()
+        for f in futures.values():
+            f.cancel()
+        return self._metrics(done, queue, pending, running, end, busy_s, su_done)
+
+    @staticmethod
+    def _metrics(done, queue, pending, running, end, busy_s, su_done):
+        lat = np.array([d[3] - d[1] for d in done]) if done else np.zeros(0)
+        wait = np.array([d[2] - d[1] for d in done]) if done else np.zeros(0)
+        ok = sum(1 for d in done if d[4] == 1)
+        # requests that arrived but did not finish: their latency is at least
+        # end - arrival (censored); folded in for a lower bound on the tail
+        unfinished = [end - r.arrival for r in list(queue) + [v[0] for v in running.values()] if r.arrival <= end]
+        allat = np.concatenate([lat, np.array(unfinished)]) if unfinished else lat
+
+        def pct(a, q):
+            return float(np.percentile(a, q)) if a.size else None
+
+        return {
+            "completed": len(done), "ok": ok, "failed": len(done) - ok,
+            "unfinished": len(unfinished), "not_arrived": len(pending),
+            "elapsed_s": end, "solves_per_s": ok / end if end > 0 else 0.0,
+            "site_updates_per_s": su_done / end if end > 0 else 0.0,
+            "latency_p50_s": pct(lat, 50), "latency_p99_s": pct(lat, 99),
+            "queue_wait_p50_s": pct(wait, 50), "queue_wait_p99_s": pct(wait, 99),
+            "latency_all_p50_s_lower_bound": pct(allat, 50), "latency_all_p99_s_lower_bound": pct(allat, 99),
+            "device_busy_frac": busy_s / end if end > 0 else 0.0,
+        }
+
+
+def main():
+    from .workloads import request_stream
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rate", type=float, default=40.0)
+    ap.add_argument("--duration", type=float, default=120.0)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--budget-gib", type=float, default=0.0)
+    ap.add_argument("--max-msites", type=float, default=64.0, help="resident grid points cap, millions")
+    ap.add_argument("--no-drain", action="store_true")
+    a = ap.parse_args()
+    reqs = request_stream(a.rate, a.duration, a.seed)
+    with Server(hbm_budget=int(a.budget_gib * 2 ** 30), max_sites=a.max_msites * 1e6) as s:
+        m = s.serve(reqs, duration=a.duration, drain=not a.no_drain,
+                    log=lambda *x: print("[serve]", *x, file=sys.stderr, flush=True))
+    m.update({"rate": a.rate, "duration": a.duration, "requests": len(reqs)})
+    print(json.dumps(m), flush=True)
+
+
+if __name__ == "__main__":
+    main()

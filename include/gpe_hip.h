@@ -35,6 +35,7 @@ extern "C" {
 #define GPE_ENONFINITE (-4) /* NaN in the adaptive error norm */
 #define GPE_ESTEP (-5)    /* adaptive step control gave up (too many rejections) */
 #define GPE_ESTATE (-6)   /* call out of order (e.g. no field uploaded) */
+#define GPE_EFULL (-7)    /* batch: the instance does not fit the HBM budget */
 
 /* scheme (run_params%scheme, solve.f90:34-50) */
 #define GPE_EULER 1
@@ -150,6 +151,62 @@ int gpe_prof_query(gpe_ctx *ctx, int32_t kind, int64_t *launches, double *total_
  * fractions are read against, beside the 8 TB/s datasheet peak. */
 int gpe_probe_bandwidth(int64_t bytes, int32_t reps, double *read_gbs, double *write_gbs,
                         double *copy_gbs);
+
+/* ---- Batches: many independent simulations on one GPU -----------------
+ *
+ * No reference counterpart: the reference runs one compile-time grid per
+ * job (parameters.in:23-29).  Each instance of a batch is a solver() loop
+ * of its own (solve.f90:25-53 as called by gpe.f90:230), with its own grid,
+ * boundary conditions, order, trap, dt and step count, in FP64 or FP32C.
+ * Per stage kind, all instances of one (precision, order, scheme) run in
+ * one persistent launch over a table of (instance, tile) work items; after
+ * the last stage a control kernel advances every instance (t, dt, the rk45
+ * accept/reject of rkqs, solve.f90:180-229) on the device.  One iteration
+ * (one rk4 step, or one rk45 attempt) is captured in a hipGraph and
+ * replayed.  An instance's results do not depend on what else is in the
+ * batch.  Instance device memory: psi, psi' and the scheme's work fields
+ * (RK4: 3, RKF45: 6) plus psi's low word (x2) in FP32C; admission is
+ * against an HBM budget.  Calls that change the set of instances
+ * synchronise the batch's stream. */
+typedef struct gpe_batch gpe_batch;
+
+/* hbm_budget: device bytes the instances may use (0 = 90% of free memory). */
+int gpe_batch_create(int64_t hbm_budget, gpe_batch **out);
+void gpe_batch_destroy(gpe_batch *b);
+/* Device bytes one instance with these parameters occupies. */
+int gpe_batch_footprint(const gpe_params *params, int64_t *bytes);
+/* Add an instance that will take nsteps solver() steps from state st and
+ * field psi (nx*ny*nz, complex128 or complex64 per host_type).  Returns
+ * GPE_EFULL, adding nothing, when it does not fit the budget. */
+int gpe_batch_add(gpe_batch *b, const gpe_params *params, const void *psi, int32_t host_type,
+                  const gpe_step_state *st, int64_t nsteps, int32_t *id);
+int gpe_batch_remove(gpe_batch *b, int32_t id);
+/* Enqueue `iterations` iterations (asynchronous).  0 only prepares: it
+ * rebuilds the work tables and captures the iteration graph if the set of
+ * instances changed. */
+int gpe_batch_run(gpe_batch *b, int32_t iterations);
+int gpe_batch_sync(gpe_batch *b);
+/* Instances that stopped since the last poll: ids and status (1 finished,
+ * -1 NaN in the rk45 error norm, -2 more than 1000 rejections); n <= cap. */
+int gpe_batch_poll(gpe_batch *b, int32_t *ids, int32_t *status, int32_t cap, int32_t *n);
+/* State of one instance: st as solver() leaves it (p = steps done),
+ * status (0 running), and get_norm of its current field. */
+int gpe_batch_query(gpe_batch *b, int32_t id, gpe_step_state *st, int64_t *steps_done, int32_t *status,
+                    double *norm);
+int gpe_batch_download(gpe_batch *b, int32_t id, void *psi, int32_t host_type);
+int gpe_batch_stats(gpe_batch *b, int32_t *instances, int32_t *running, int64_t *bytes_in_use,
+                    int64_t *budget);
+/* 1: replay the captured iteration graph (default); 0: launch kernels. */
+int gpe_batch_use_graph(gpe_batch *b, int32_t enable);
+/* Tile variant and z-chunk (0 = whole column) of one stage kind for the
+ * instances of (precision, order, scheme). */
+int gpe_batch_tune_set(gpe_batch *b, int32_t precision, int32_t order, int32_t scheme, int32_t kind,
+                       int32_t variant, int32_t zchunk);
+int gpe_batch_tune_get(gpe_batch *b, int32_t precision, int32_t order, int32_t scheme, int32_t kind,
+                       int32_t *variant, int32_t *zchunk);
+/* Time every variant and z-chunk of every stage kind over the current
+ * instances and keep the fastest.  Commits no step. */
+int gpe_batch_autotune(gpe_batch *b, int32_t reps);
 
 #ifdef __cplusplus
 }
