@@ -209,12 +209,17 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-simulation batch leg")
     ap.add_argument("--serve-seconds", type=float, default=30.0,
                     help="length of the Poisson serving leg (0 to skip)")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the batch leg and print its JSON (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     from gpe_amd import _lib, ic
+    if a.only_batch:
+        print(json.dumps({"batch": batch_leg(a.seed)}), flush=True)
+        return
     probe = _lib.probe_bandwidth(1 << 31, 20)
     log("HBM probe GB/s: read %.0f write %.0f copy %.0f" % (probe["read"], probe["write"], probe["copy"]))
     t0 = time.perf_counter()

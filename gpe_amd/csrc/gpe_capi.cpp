@@ -117,10 +117,12 @@ int grid_for_v(gpe_ctx *c, int vid, int zc, StageArgs &A) {
   const int tiles2d = A.ntx * A.nty;
   if (zc <= 0) {
     // enough workgroups to fill 256 CUs several times over, but z-chunks
-    // long enough that re-reading 2R planes per chunk stays small
+    // long enough that re-reading 2R planes per chunk stays small: at least
+    // 32 planes unless that leaves fewer than 1024 workgroups (small grids,
+    // where the z march's latency, not its traffic, sets the time)
     int nzc = (2048 + tiles2d - 1) / tiles2d;
     zc = (c->P.nz + nzc - 1) / nzc;
-    if (zc < 32) zc = 32;
+    if (zc < 32 && tiles2d * ((c->P.nz + 31) / 32) >= 1024) zc = 32;
   }
   if (zc < 8) zc = 8;
   if (zc > c->P.nz) zc = c->P.nz;

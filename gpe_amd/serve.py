@@ -152,7 +152,13 @@ class Server:
         end = clock()
         for f in futures.values():
             f.cancel()
-        return self._metrics(done, queue, pending, running, end, busy_s, su_done)
+        # work done on requests still running counts toward throughput
+        su_partial = 0.0
+        for iid, (r, _, sites) in running.items():
+            su_partial += float(sites) * self.batch.query(iid)["steps"]
+        m = self._metrics(done, queue, pending, running, end, busy_s, su_done + su_partial)
+        m["site_updates_per_s_completed_only"] = su_done / end if end > 0 else 0.0
+        return m
 
     @staticmethod
     def _metrics(done, queue, pending, running, end, busy_s, su_done):
