@@ -28,7 +28,7 @@ EXPORTS = [
     "gpe_last_error", "gpe_ctx_create", "gpe_ctx_destroy", "gpe_solver_step",
     "gpe_sim_upload", "gpe_sim_download", "gpe_sim_set_state", "gpe_sim_get_state",
     "gpe_sim_advance", "gpe_sim_norm_history", "gpe_sim_norm", "gpe_sim_energy",
-    "gpe_sim_mass", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
+    "gpe_sim_mass", "gpe_sim_linelength", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
     "gpe_tune_set", "gpe_tune_get", "gpe_autotune", "gpe_probe_bandwidth",
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_footprint", "gpe_batch_add",
     "gpe_batch_remove", "gpe_batch_run", "gpe_batch_sync", "gpe_batch_poll", "gpe_batch_query",
@@ -95,6 +95,7 @@ def lib():
         "gpe_sim_norm": ([vp, dp], ctypes.c_int),
         "gpe_sim_energy": ([vp, dp], ctypes.c_int),
         "gpe_sim_mass": ([vp, dp], ctypes.c_int),
+        "gpe_sim_linelength": ([vp, dp], ctypes.c_int),
         "gpe_sim_renormalise": ([vp, ctypes.c_double, ctypes.c_double], ctypes.c_int),
         "gpe_sim_rhs": ([vp, vp, i32], ctypes.c_int),
         "gpe_prof_enable": ([vp, i32], ctypes.c_int),

@@ -1,0 +1,134 @@
+"""Accuracy of the compensated FP32 form against the FP64 path on the GPU.
+
+    python -m gpe_amd.accuracy [--n 512] [--steps 1000]
+
+Runs the random-phase periodic field (dx = 0.5, 4th-order FD, RKF45 with
+tolerance 1e-6, V = 0, g = mu = 1) for `steps` accepted steps in FP64 and
+in FP32C from the same initial state, and reports for each: simulated time,
+the reference's energy functional (energy, variables.f90:580-604), the
+eqn-4 Hamiltonian H = K + (g/2)int|psi|^4 - mu N that the dynamics conserve,
+the norm, and the vortex-line length (linelength, variables.f90:896-1162);
+then the relative L2 distance of the two fields and the differences of
+those quantities.
+
+With periodic walls int Re(-lap(psi) psi*) = 2K, so the reference functional
+E_ref = 2K + P (P = (1/2) int |psi|^4) gives H = (E_ref + P)/2 - mu N; P and N
+are summed on the host from the downloaded field.
+
+Each step of the FP32C run starts from the FP64 run's time and proposed dt
+(its own step control still accepts or rejects), so both fields are
+compared at the same times; the number of steps where the two controls
+ended at different times is reported (0 means every accepted dt matched).
+The random-phase tangle is chaotic, so a control run measures how much any
+perturbation grows: FP64 again, from the initial field rounded to
+complex64 (a relative change of ~3e-8), on the same time sequence.
+
+A second report covers the 64^3 vortex ring (dx = 0.4, reflective, 2nd
+order, RK4 dt = 0.01, 1000 steps): FP32C against FP64 after 1000 RK4 steps.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import ic
+from .config import RunConfig, xr_for_spacing
+from .sim import Simulation
+
+
+def workload(precision, n):
+    return RunConfig(nx=n, ny=n, nz=n, xr=xr_for_spacing(0.5, n), bcs=1, order=4, scheme="rk45", eqn=4, g=1.0,
+                     mu=1.0, tau=0.05, precision=precision)
+
+
+def _observables(sim, cfg):
+    psi = sim.download(np.complex128)
+    dv = cfg.dx * cfg.dy * cfg.dz
+    a2 = psi.real * psi.real + psi.imag * psi.imag
+    N = float(a2.sum()) * dv
+    P = 0.5 * float((a2 * a2).sum()) * dv
+    E = sim.energy()
+    H = 0.5 * (E + P) - cfg.mu * N
+    return psi, {"t": sim.state()["t"], "energy_ref": E, "hamiltonian": H, "norm": sim.norm(), "mass": N,
+                 "linelength": sim.linelength()}
+
+
+def run(n=512, steps=1000, seed=20261202, log=None):
+    log = log or (lambda *a: None)
+    psi0 = ic.random_phase(workload("fp64", n), seed)
+    out = {"grid": n, "steps": steps, "seed": seed}
+    c64, c32 = workload("fp64", n), workload("fp32c", n)
+    with Simulation(c64, psi0) as s64, Simulation(c32, psi0) as s32, \
+            Simulation(c64, psi0.astype(np.complex64).astype(np.complex128)) as sct:
+        del psi0
+        _, o64 = _observables(s64, c64)
+        _, o32 = _observables(s32, c32)
+        start = {"fp64": o64, "fp32c": o32}
+        diverged = 0
+        t0 = time.perf_counter()
+        for i in range(steps):
+            st = s64.state()
+            for s in (s32, sct):
+                s.set_state(st["dt"], t=st["t"], im_t=st["im_t"], p=i)
+                s.advance(1)
+            s64.advance(1)
+            if s32.state()["t"] != s64.state()["t"]:
+                diverged += 1
+            if (i + 1) % 100 == 0:
+                log("step %d, t = %.6f, %.1f s" % (i + 1, s64.state()["t"], time.perf_counter() - t0))
+        p64, e64 = _observables(s64, c64)
+        p32, e32 = _observables(s32, c32)
+        pct = sct.download()
+    out["psi_rel_l2_control_fp64_from_rounded_ic"] = float(np.linalg.norm((pct - p64).ravel()) /
+                                                           np.linalg.norm(p64.ravel()))
+    del pct
+    for prec, e in (("fp64", e64), ("fp32c", e32)):
+        o = start[prec]
+        out[prec] = {"start": o, "end": e,
+                     "energy_ref_drift": (e["energy_ref"] - o["energy_ref"]) / abs(o["energy_ref"]),
+                     "hamiltonian_drift": (e["hamiltonian"] - o["hamiltonian"]) / abs(o["hamiltonian"]),
+                     "norm_drift": (e["norm"] - o["norm"]) / o["norm"]}
+    out["steps_with_different_dt"] = diverged
+    out["psi_rel_l2"] = float(np.linalg.norm((p32 - p64).ravel()) / np.linalg.norm(p64.ravel()))
+    out["linelength_rel_diff"] = (e32["linelength"] - e64["linelength"]) / e64["linelength"]
+    out["hamiltonian_rel_diff"] = (e32["hamiltonian"] - e64["hamiltonian"]) / abs(e64["hamiltonian"])
+    out["energy_ref_rel_diff"] = (e32["energy_ref"] - e64["energy_ref"]) / abs(e64["energy_ref"])
+    return out
+
+
+def ring_rk4(steps=1000):
+    """64^3 ring, RK4: FP32C against FP64 after `steps` steps."""
+    cfg = RunConfig(nx=64, ny=64, nz=64, xr=xr_for_spacing(0.4, 64), bcs=2, order=2, scheme="rk4", tau=0.01,
+                    g=1.0, mu=1.0)
+    psi0 = ic.vortex_ring(cfg, 4.0)
+    res = {}
+    for prec in ("fp64", "fp32c"):
+        c = cfg.with_(precision=prec)
+        with Simulation(c, psi0) as sim:
+            e0 = sim.energy()
+            sim.advance(steps)
+            res[prec] = (sim.download(np.complex128), sim.energy(), e0, sim.linelength())
+    a, b = res["fp32c"], res["fp64"]
+    return {"steps": steps, "psi_rel_l2": float(np.linalg.norm((a[0] - b[0]).ravel()) / np.linalg.norm(b[0].ravel())),
+            "energy_ref_rel_diff": (a[1] - b[1]) / abs(b[1]),
+            "energy_ref_drift": {"fp64": (b[1] - b[2]) / abs(b[2]), "fp32c": (a[1] - a[2]) / abs(a[2])},
+            "linelength": {"fp64": b[3], "fp32c": a[3]}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--seed", type=int, default=20261202)
+    a = ap.parse_args()
+    res = {"ring_64_rk4": ring_rk4()}
+    print("[accuracy] ring:", json.dumps(res["ring_64_rk4"]), file=sys.stderr, flush=True)
+    res["random_phase_rk45"] = run(a.n, a.steps, a.seed,
+                                   log=lambda *x: print("[accuracy]", *x, file=sys.stderr, flush=True))
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -6,6 +6,7 @@
 // solver(var_in, var_out) call (solve.f90:25-53).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -104,6 +105,9 @@ struct gpe_ctx {
   std::vector<Ev> pending;
   std::vector<hipEvent_t> pool;
   int64_t prof_n[GPE_PROF_KINDS] = {};
+  // line-length point lists (grown on demand)
+  void *ll_buf = nullptr;
+  int ll_cap = 0;
   double prof_ms[GPE_PROF_KINDS] = {};
 };
 
@@ -371,7 +375,7 @@ int solver_call(gpe_ctx *c, double *hist_slot) {
 
 void free_all(gpe_ctx *c) {
   void *ptrs[] = {c->psi, c->psi2, c->lo, c->lo2, c->k[0], c->k[1], c->k[2], c->k[3], c->k[4], c->k[5],
-                  c->acc, c->scale, c->part, c->dscal, c->hist};
+                  c->acc, c->scale, c->part, c->dscal, c->hist, c->ll_buf};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->hscal) hipHostFree(c->hscal);
@@ -587,6 +591,89 @@ int gpe_sim_energy(gpe_ctx *c, double *energy) {
   HIPCHK(hipMemcpyAsync(c->hscal, c->dscal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   *energy = c->hscal[2];
+  return GPE_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Segment of a one-point cell to the latest earlier second point, with the
+// reference's face/edge weights (variables.f90:1140-1153), in T.
+template <typename T>
+T ll_segment_host(T x1, T y1, T z1, T x2, T y2, T z2) {
+  auto in01 = [](T v) { return v == (T)0 || v == (T)1; };
+  T a = (T)1;
+  const bool ex = x1 == x2 && in01(x1), ey = y1 == y2 && in01(y1), ez = z1 == z2 && in01(z1);
+  if (ex || ey) {
+    a = a * (T)0.5;
+    if ((ex && ey) || (ey && ez)) a = a * (T)0.5;
+  }
+  return a * std::sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2) + (z1 - z2) * (z1 - z2));
+}
+
+template <typename T>
+double ll_singles(std::vector<LinePoint> &w, std::vector<LinePoint> &s) {
+  auto by_cell = [](const LinePoint &a, const LinePoint &b) { return a.cell < b.cell; };
+  std::sort(w.begin(), w.end(), by_cell);
+  std::sort(s.begin(), s.end(), by_cell);
+  double sum = 0.0;
+  T x2 = 0, y2 = 0, z2 = 0;  // before any cell stored a second point
+  size_t j = 0;
+  for (const LinePoint &p : s) {
+    while (j < w.size() && w[j].cell < p.cell) {
+      x2 = (T)w[j].x;
+      y2 = (T)w[j].y;
+      z2 = (T)w[j].z;
+      ++j;
+    }
+    sum += (double)ll_segment_host<T>((T)p.x, (T)p.y, (T)p.z, x2, y2, z2);
+  }
+  return sum;
+}
+}  // namespace
+
+extern "C" {
+
+int gpe_sim_linelength(gpe_ctx *c, double *length) {
+  if (!c || !length) return fail(GPE_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  const long long ncell = (long long)(c->P.nx - 2) * (c->P.ny - 2) * (c->P.nz - 2);
+  const int nb = (int)std::min<long long>((ncell + 255) / 256, std::min(c->part_cap, 2048));
+  int cnt[2] = {0, 0};
+  for (;;) {
+    if (!c->ll_buf || c->ll_cap < 1) {
+      const int cap = c->ll_cap > 0 ? c->ll_cap : (1 << 16);
+      if (c->ll_buf) hipFree(c->ll_buf);
+      c->ll_buf = nullptr;
+      if (hipMalloc(&c->ll_buf, 2 * sizeof(LinePoint) * (size_t)cap + 256) != hipSuccess)
+        return fail(GPE_ENOMEM, "line-length lists of %d points", cap);
+      c->ll_cap = cap;
+    }
+    LinePoint *w = (LinePoint *)c->ll_buf, *sg = w + c->ll_cap;
+    int *counts = (int *)(sg + c->ll_cap);
+    HIPCHK(hipMemsetAsync(counts, 0, 2 * sizeof(int), c->stream));
+    hipError_t e = launch_linelength(c->prec, c->psi, c->comp ? c->lo : nullptr, c->P.nx, c->P.ny, c->P.nz, c->part,
+                                     nb, w, sg, counts, c->ll_cap, c->stream);
+    if (e != hipSuccess) return fail(GPE_EHIP, "linelength launch: %s", hipGetErrorString(e));
+    HIPCHK(hipMemcpyAsync(cnt, counts, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int need = std::max(cnt[0], cnt[1]);
+    if (need <= c->ll_cap) break;
+    // lists overflowed: grow them and run the pass again
+    hipFree(c->ll_buf);
+    c->ll_buf = nullptr;
+    c->ll_cap = need + need / 4;
+  }
+  std::vector<double> part(nb);
+  std::vector<LinePoint> w(cnt[0]), sg(cnt[1]);
+  LinePoint *dw = (LinePoint *)c->ll_buf;
+  HIPCHK(hipMemcpy(part.data(), c->part, sizeof(double) * nb, hipMemcpyDeviceToHost));
+  if (cnt[0]) HIPCHK(hipMemcpy(w.data(), dw, sizeof(LinePoint) * cnt[0], hipMemcpyDeviceToHost));
+  if (cnt[1]) HIPCHK(hipMemcpy(sg.data(), dw + c->ll_cap, sizeof(LinePoint) * cnt[1], hipMemcpyDeviceToHost));
+  double sum = 0.0;
+  for (double v : part) sum += v;
+  sum += c->prec == GPE_FP32 ? ll_singles<float>(w, sg) : ll_singles<double>(w, sg);
+  *length = c->prec == GPE_FP32 ? (double)((float)sum * (float)c->P.dx) : sum * c->P.dx;
   return GPE_OK;
 }
 

@@ -1,6 +1,7 @@
 // gpe_misc.hip -- reductions and pointwise kernels, and the dispatch over
 // the stage-kernel variants.  Built with -ffp-contract=off.
 #include "batch_ctl.hpp"
+#include "diag.hpp"
 #include "launch.h"
 #include "pointwise.hpp"
 
@@ -54,6 +55,21 @@ hipError_t launch_batch_control(InstCtl *ctl, StageArgs *args, int cap, int *act
                                 const double *pn, const double *pe, hipStream_t s) {
   if (cap <= 0) return hipSuccess;
   hipLaunchKernelGGL(batch_control, dim3(cap), dim3(64), 0, s, ctl, args, cap, active, range, pn, pe);
+  return hipGetLastError();
+}
+
+hipError_t launch_linelength(int prec, const void *psi, const void *lo, int nx, int ny, int nz, double *part,
+                             int nblocks, LinePoint *writers, LinePoint *singles, int *counts, int cap,
+                             hipStream_t s) {
+  if (prec == 1)  // GPE_FP32: the reference built in single precision
+    hipLaunchKernelGGL((linelength_cells<float, float>), dim3(nblocks), dim3(256), 0, s, (const float *)psi,
+                       (const float *)nullptr, nx, ny, nz, part, writers, singles, counts, cap);
+  else if (prec == 2)
+    hipLaunchKernelGGL((linelength_cells<float, double>), dim3(nblocks), dim3(256), 0, s, (const float *)psi,
+                       (const float *)lo, nx, ny, nz, part, writers, singles, counts, cap);
+  else
+    hipLaunchKernelGGL((linelength_cells<double, double>), dim3(nblocks), dim3(256), 0, s, (const double *)psi,
+                       (const double *)nullptr, nx, ny, nz, part, writers, singles, counts, cap);
   return hipGetLastError();
 }
 

@@ -50,6 +50,12 @@ hipError_t launch_norm_f32(const void *psi, const void *lo, int nx, int ny, int 
 hipError_t launch_renorm_f64(void *psi, size_t n, double a, DivC b, hipStream_t s);
 hipError_t launch_renorm_f32(void *psi, void *lo, size_t n, double a, double b, hipStream_t s);
 
+// Vortex-line length cells (diag.hpp): prec GPE_FP64 / GPE_FP32 / GPE_FP32C
+// (lo = low word for the compensated form); nblocks partial sums into part.
+hipError_t launch_linelength(int prec, const void *psi, const void *lo, int nx, int ny, int nz, double *part,
+                             int nblocks, LinePoint *writers, LinePoint *singles, int *counts, int cap,
+                             hipStream_t s);
+
 // HBM read / write / copy bandwidth (GB/s) of float4 arrays of `bytes`.
 hipError_t probe_bandwidth(size_t bytes, int reps, double *gbs);
 

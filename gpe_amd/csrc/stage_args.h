@@ -105,4 +105,10 @@ struct InstCtl {
   void *buf[2], *lob[2];
 };
 
+// A zero point of the vortex-line length pass (see diag.hpp).
+struct LinePoint {
+  long long cell;  // linear cell index, x fastest
+  double x, y, z;
+};
+
 }  // namespace gpe

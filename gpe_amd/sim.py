@@ -143,6 +143,10 @@ class Simulation:
     def mass(self):
         return self._scalar(self._L.gpe_sim_mass)
 
+    def linelength(self):
+        """Total vortex-line length (linelength, variables.f90:896-1162)."""
+        return self._scalar(self._L.gpe_sim_linelength)
+
     def renormalise(self, nn, norm):
         _lib.check(self._L.gpe_sim_renormalise(self._ctx, float(nn), float(norm)))
 

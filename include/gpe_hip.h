@@ -118,6 +118,10 @@ int gpe_sim_norm_history(gpe_ctx *ctx, double *out, int64_t n);
 int gpe_sim_norm(gpe_ctx *ctx, double *norm);     /* get_norm */
 int gpe_sim_energy(gpe_ctx *ctx, double *energy); /* energy */
 int gpe_sim_mass(gpe_ctx *ctx, double *mass);     /* mass */
+/* linelength (variables.f90:896-1162): total vortex-line length, the
+ * reference's cell-by-cell zero tracing (including its reuse of the previous
+ * cell's second point in cells that find a single point). */
+int gpe_sim_linelength(gpe_ctx *ctx, double *length);
 /* renormalise (variables.f90:715-726): psi = sqrt(nn)*psi/sqrt(norm). */
 int gpe_sim_renormalise(gpe_ctx *ctx, double nn, double norm);
 /* f(psi) of the resident field into a host buffer (get_rhs, solve.f90:382). */

@@ -23,6 +23,7 @@
  *   get_norm          variables.f90:554-576, integrate_x/y/z 730-840
  *   energy / mass     variables.f90:580-628
  *   renormalise       variables.f90:715-726
+ *   linelength        variables.f90:896-1162
  *
  * Complex arithmetic follows what this image's flang (the compiler the
  * reference is built with here) emits for the reference's expressions,
@@ -696,4 +697,134 @@ void FN(renormalise)(const orc_params *p, REAL *f, double nn, double norm) {
   long n = (long)p->nx * p->ny * p->nz;
   REAL a = r_sqrt((REAL)nn), b = r_sqrt((REAL)norm);
   for (long q = 0; q < n; q++) st(f, q, cx_rdiv(cx_scale(a, ld(f, q)), b));
+}
+
+/* ---- linelength (variables.f90:896-1162) ---------------------------------
+ * Total vortex-line length: every interior cell (corners i..i+1, j..j+1,
+ * k..k+1 with 1 <= i,j,k <= n-2) is tested face by face for a point where
+ * the zero lines of Re(psi) and Im(psi) -- each traced as the segment
+ * between its crossings of the face's edges -- intersect inside the face.
+ * Two such points give a segment whose length is added (halved when it
+ * lies in a face shared with a neighbouring cell, quartered on an edge); a
+ * face that is entirely a zero of one component with a sign change of the
+ * other adds 1/2 and ends the cell, and an edge that is a zero of both adds
+ * 1/4 and ends the cell.  The result is scaled by dx.
+ *
+ * The point arrays of the reference keep their values from cell to cell, so
+ * a cell in which only ONE point is found measures from that point to the
+ * second point of the most recent earlier cell (in x-fastest order) that
+ * stored one; this restatement keeps that behaviour, with (0, 0, 0) before
+ * any cell has stored a second point. */
+typedef struct {
+  REAL x[7], y[7], z[7];
+} ll_pts;
+
+/* Corner offsets (di, dj, dk) of u(1..4) for faces 1..6: front (k), back
+ * (k+1), right (i+1), left (i), top (j+1), bottom (j). */
+static const int ll_corner[6][4][3] = {
+    {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}}, {{0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}},
+    {{1, 0, 0}, {1, 0, 1}, {1, 1, 1}, {1, 1, 0}}, {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}},
+    {{0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}}, {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}}};
+
+/* Edge crossings of one component w(1..5) (w5 = w1) of a face, in face
+ * coordinates; returns the count index the reference calls lu / lv. */
+static int ll_crossings(const REAL w[6], REAL *xs, REAL *ys) {
+  int l = 1;
+  for (int q = 1; q <= 4; q++) {
+    if (w[q] == (REAL)0.0) {
+      /* corner q; the index is not advanced (variables.f90:1025-1037) */
+      static const REAL cxq[5] = {0, 0, 1, 1, 0}, cyq[5] = {0, 0, 0, 1, 1};
+      xs[l] = cxq[q];
+      ys[l] = cyq[q];
+    } else if (w[q] * w[q + 1] < (REAL)0.0) {
+      switch (q) {
+        case 1: xs[l] = r_abs(w[1] / (w[1] - w[2])); ys[l] = (REAL)0.0; break;
+        case 2: xs[l] = (REAL)1.0; ys[l] = r_abs(w[2] / (w[2] - w[3])); break;
+        case 3: xs[l] = r_abs(w[4] / (w[3] - w[4])); ys[l] = (REAL)1.0; break;
+        default: xs[l] = (REAL)0.0; ys[l] = r_abs(w[5] / (w[4] - w[5])); break;
+      }
+      if (l == 1 || xs[l] != xs[l - 1] || ys[l] != ys[l - 1]) l++;
+    }
+  }
+  return l;
+}
+
+static int ll_in01(REAL v) { return v == (REAL)0.0 || v == (REAL)1.0; }
+
+double FN(linelength)(const orc_params *p, const REAL *f) {
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  ll_pts P;
+  memset(&P, 0, sizeof P);
+  REAL L = (REAL)0.0;
+  for (int k = 1; k <= nz - 2; k++)
+    for (int j = 1; j <= ny - 2; j++)
+      for (int i = 1; i <= nx - 2; i++) {
+        int m = 1, done = 0;
+        for (int n = 0; n < 6 && !done; n++) {
+          REAL u[6], v[6];
+          for (int q = 0; q < 4; q++) {
+            const int *o = ll_corner[n][q];
+            const long s = ((long)(k + o[2]) * ny + (j + o[1])) * nx + (i + o[0]);
+            u[q + 1] = f[2 * s];
+            v[q + 1] = f[2 * s + 1];
+          }
+          u[5] = u[1];
+          v[5] = v[1];
+          const int uz = u[1] == 0 && u[2] == 0 && u[3] == 0 && u[4] == 0;
+          const int vz = v[1] == 0 && v[2] == 0 && v[3] == 0 && v[4] == 0;
+          const int vs = v[1] * v[2] < 0 || v[2] * v[3] < 0 || v[3] * v[4] < 0 || v[4] * v[1] < 0;
+          const int us = u[1] * u[2] < 0 || u[2] * u[3] < 0 || u[3] * u[4] < 0 || u[4] * u[1] < 0;
+          if ((uz && vs) || (vz && us)) {  /* a face of zeros (variables.f90:981-993) */
+            L = L + (REAL)0.5;
+            done = 1;
+            break;
+          }
+          for (int q = 1; q <= 4 && !done; q++)
+            if (u[q] == 0 && u[q + 1] == 0 && v[q] == 0 && v[q + 1] == 0) {  /* an edge (995-1002) */
+              L = L + (REAL)0.25;
+              done = 1;
+            }
+          if (done) break;
+          REAL xu[7], yu[7], xv[7], yv[7];
+          const int lu = ll_crossings(u, xu, yu);
+          const int lv = ll_crossings(v, xv, yv);
+          if (lu > 2 && lv > 2) {
+            REAL den = xv[2] * (yu[1] - yu[2]) + xv[1] * (yu[2] - yu[1]) + (xu[1] - xu[2]) * (yv[1] - yv[2]);
+            if (den == (REAL)0.0) den = den + (REAL)0.0000001;
+            const REAL xx = (xu[1] * (xv[2] * (yv[1] - yu[2]) + xv[1] * (yu[2] - yv[2])) +
+                             xu[2] * (xv[2] * (yu[1] - yv[1]) + xv[1] * (yv[2] - yu[1]))) / den;
+            const REAL yy = (xv[2] * (yu[1] - yu[2]) * yv[1] + xu[1] * yu[2] * yv[1] - xv[1] * yu[1] * yv[2] -
+                             xu[1] * yu[2] * yv[2] + xv[1] * yu[2] * yv[2] + xu[2] * yu[1] * (yv[2] - yv[1])) / den;
+            if (xx >= 0 && xx <= 1 && yy >= 0 && yy <= 1 && m <= 6) {
+              REAL px, py, pz;
+              switch (n) {
+                case 0: px = xx; py = yy; pz = 0; break;
+                case 1: px = xx; py = yy; pz = 1; break;
+                case 2: px = 1; py = yy; pz = xx; break;
+                case 3: px = 0; py = yy; pz = xx; break;
+                case 4: px = xx; py = 1; pz = yy; break;
+                default: px = xx; py = 0; pz = yy; break;
+              }
+              P.x[m] = px;
+              P.y[m] = py;
+              P.z[m] = pz;
+              m++;
+            }
+          }
+        }
+        if (!done && m > 1) {
+          REAL a = (REAL)1.0;
+          const int ex = P.x[1] == P.x[2] && ll_in01(P.x[1]);
+          const int ey = P.y[1] == P.y[2] && ll_in01(P.y[1]);
+          const int ez = P.z[1] == P.z[2] && ll_in01(P.z[1]);
+          if (ex || ey) {  /* (the reference tests y twice, z never, here) */
+            a = a * (REAL)0.5;
+            if ((ex && ey) || (ey && ez)) a = a * (REAL)0.5;
+          }
+          const REAL d = r_sqrt((P.x[1] - P.x[2]) * (P.x[1] - P.x[2]) + (P.y[1] - P.y[2]) * (P.y[1] - P.y[2]) +
+                                (P.z[1] - P.z[2]) * (P.z[1] - P.z[2]));
+          L = L + a * d;
+        }
+      }
+  return (double)(L * (REAL)p->dx);
 }

@@ -55,7 +55,8 @@ typedef struct orc_state {
   double P##_norm(const orc_params *p, const REAL *f);                      \
   double P##_energy(const orc_params *p, const REAL *f);                    \
   double P##_mass(const orc_params *p, const REAL *f);                      \
-  void P##_renormalise(const orc_params *p, REAL *f, double nn, double norm);
+  void P##_renormalise(const orc_params *p, REAL *f, double nn, double norm); \
+  double P##_linelength(const orc_params *p, const REAL *f);
 
 ORC_DECL(orc64, double)
 ORC_DECL(orc32, float)

@@ -62,7 +62,7 @@ def lib():
             f.argtypes = [P, ctypes.POINTER(OrcState), vp, vp]; f.restype = I
             f = getattr(_LIB, pfx + "_run")
             f.argtypes = [P, ctypes.POINTER(OrcState), vp, I, vp]; f.restype = I
-            for nm in ("_norm", "_energy", "_mass"):
+            for nm in ("_norm", "_energy", "_mass", "_linelength"):
                 f = getattr(_LIB, pfx + nm); f.argtypes = [P, vp]; f.restype = D
             f = getattr(_LIB, pfx + "_renormalise"); f.argtypes = [P, vp, D, D]; f.restype = None
     return _LIB
@@ -175,3 +175,10 @@ def renormalise(cfg, psi, nn, nrm):
     out = np.ascontiguousarray(psi).copy()
     getattr(lib(), _pfx(out) + "_renormalise")(ctypes.byref(params(cfg, out.shape)), _ptr(out), float(nn), float(nrm))
     return out
+
+
+def linelength(cfg, psi):
+    """linelength (variables.f90:896-1162) of the field, in the field's
+    precision."""
+    psi = np.ascontiguousarray(psi)
+    return getattr(lib(), _pfx(psi) + "_linelength")(ctypes.byref(params(cfg, psi.shape)), _ptr(psi))

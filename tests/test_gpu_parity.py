@@ -252,6 +252,15 @@ def test_config0_ring_rk4_1000_steps(gpu, oracle):
     assert err < 1e-12
     assert abs(gn[-1] - wnorms[-1]) / wnorms[-1] < 1e-12
     assert ge == pytest.approx(oracle.energy(cfg, want, NT), rel=1e-12)
+    with Simulation(cfg, psi) as sim:
+        assert sim.linelength() == pytest.approx(oracle.linelength(cfg, psi), rel=1e-12)
+    # the compensated FP32 form after the same 1000 RK4 steps: <= 1e-5
+    with Simulation(cfg.with_(precision="fp32c"), psi) as sim:
+        sim.advance(1000)
+        got32 = sim.download(np.complex128)
+    err32 = rel_l2(got32, want)
+    say("config 0 fp32c rel L2 = %.3e" % err32)
+    assert err32 < 1e-5
 
 
 def test_config1_line_pair_256(gpu, oracle):
