@@ -82,8 +82,13 @@ def run_path(precision, psi, steps, warmup, tune=True):
         "rejects_last": s2["n_rejects"],
         "sites": sites,
         "tiles": tiles,
-        "attempts_per_step": prof.get("ck_6", (0,))[0] / max(2, min(steps, 10)),
+        "attempts_per_step": _attempts(prof) / max(2, min(steps, 10)),
     }
+
+
+def _attempts(prof):
+    # one final-stage launch per rk45 attempt (fused: the ck_56 pair)
+    return prof.get("ck_6", prof.get("ck_56", (0,)))[0]
 
 
 def measured_traffic():
@@ -113,7 +118,7 @@ def roofline_of(res, precision):
 def step_roofline(res, bytes_per_site_attempt):
     # whole-step achieved bandwidth over all stage kernels
     tot_ms = sum(v[1] for v in res["prof"].values())
-    attempts = res["prof"].get("ck_6", (0,))[0]
+    attempts = _attempts(res["prof"])
     if not attempts:
         return None
     bytes_ = bytes_per_site_attempt * res["sites"] * attempts

@@ -14,14 +14,18 @@ GPE_EFULL = -7
 GPE_EULER, GPE_RK2, GPE_RK4, GPE_RK45 = 1, 2, 4, 45
 GPE_FP64, GPE_FP32, GPE_FP32C = 0, 1, 2
 GPE_HOST_C128, GPE_HOST_C64 = 0, 1
-GPE_PROF_KINDS = 16
+GPE_PROF_KINDS = 24
 
 SCHEME_CODES = {"euler": GPE_EULER, "rk2": GPE_RK2, "rk4": GPE_RK4, "rk45": GPE_RK45}
 PRECISION_CODES = {"fp64": GPE_FP64, "fp32": GPE_FP32, "fp32c": GPE_FP32C}
 
 # Stage kinds, in the order of gpe::Kind (csrc/stage_args.h).
 KIND_NAMES = ["rhs", "euler", "rk2_1", "rk2_2", "rk4_1", "rk4_2", "rk4_3", "rk4_4",
-              "ck_1", "ck_2", "ck_3", "ck_4", "ck_5", "ck_6", "energy"]
+              "ck_1", "ck_2", "ck_3", "ck_4", "ck_5", "ck_6", "energy", "-",
+              # fused stage pairs (ids 16..20)
+              "rk4_12", "rk4_34", "ck_12", "ck_34", "ck_56"]
+STAGE_KINDS = KIND_NAMES[:15]
+PAIR_KINDS = KIND_NAMES[16:]
 
 # Exported entry points (must match include/gpe_hip.h).
 EXPORTS = [
@@ -29,7 +33,7 @@ EXPORTS = [
     "gpe_sim_upload", "gpe_sim_download", "gpe_sim_set_state", "gpe_sim_get_state",
     "gpe_sim_advance", "gpe_sim_norm_history", "gpe_sim_norm", "gpe_sim_energy",
     "gpe_sim_mass", "gpe_sim_linelength", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
-    "gpe_tune_set", "gpe_tune_get", "gpe_autotune", "gpe_probe_bandwidth",
+    "gpe_tune_set", "gpe_set_fusion", "gpe_tune_get", "gpe_autotune", "gpe_probe_bandwidth",
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_footprint", "gpe_batch_add",
     "gpe_batch_remove", "gpe_batch_run", "gpe_batch_sync", "gpe_batch_poll", "gpe_batch_query",
     "gpe_batch_download", "gpe_batch_stats", "gpe_batch_use_graph", "gpe_batch_tune_set",
@@ -101,6 +105,7 @@ def lib():
         "gpe_prof_enable": ([vp, i32], ctypes.c_int),
         "gpe_prof_query": ([vp, i32, ctypes.POINTER(i64), dp, dp, ctypes.c_char_p], ctypes.c_int),
         "gpe_tune_set": ([vp, i32, i32, i32], ctypes.c_int),
+        "gpe_set_fusion": ([vp, i32], ctypes.c_int),
         "gpe_tune_get": ([vp, i32] + [ctypes.POINTER(i32)] * 4, ctypes.c_int),
         "gpe_autotune": ([vp, i32], ctypes.c_int),
         "gpe_probe_bandwidth": ([i64, i32, dp, dp, dp], ctypes.c_int),

@@ -22,6 +22,7 @@
 using namespace gpe;
 
 
+
 namespace {
 
 #define HIPCHK(call)                                                                        \
@@ -35,10 +36,16 @@ namespace {
 
 namespace {
 
+// Profiling / tuning ids: stage kinds 0..K_COUNT-1, fused pairs KP0 + pair.
+constexpr int KP0 = 16;
+
 const char *kind_name(int k) {
   static const char *names[] = {"rhs",   "euler", "rk2_1", "rk2_2", "rk4_1", "rk4_2", "rk4_3", "rk4_4",
                                 "ck_1",  "ck_2",  "ck_3",  "ck_4",  "ck_5",  "ck_6",  "energy"};
-  return (k >= 0 && k < K_COUNT) ? names[k] : "?";
+  static const char *pairs[] = {"rk4_12", "rk4_34", "ck_12", "ck_34", "ck_56"};
+  if (k >= 0 && k < K_COUNT) return names[k];
+  if (k >= KP0 && k < KP0 + N_PAIR_KINDS) return pairs[k - KP0];
+  return "?";
 }
 
 // Fields read + written per site by each kind in the fused-sweep traffic
@@ -63,8 +70,18 @@ int kind_passes(int k) {
     case K_CK_5: return 6;
     case K_CK_6: return 7;
     case K_ENERGY: return 1;
+    // fused pairs: the two stages' shares of the model
+    case KP0 + P_RK4_12: return 2 + 4;
+    case KP0 + P_RK4_34: return 3 + 4;
+    case KP0 + P_CK_12: return 2 + 3;
+    case KP0 + P_CK_34: return 4 + 5;
+    case KP0 + P_CK_56: return 6 + 7;
     default: return 0;
   }
+}
+
+bool kind_final(int k) {
+  return k == K_EULER || k == K_RK2_2 || k == K_RK4_4 || k == K_CK_6 || k == KP0 + P_RK4_34 || k == KP0 + P_CK_56;
 }
 
 }  // namespace
@@ -96,6 +113,11 @@ struct gpe_ctx {
   // and z-chunk length (0 = automatic)
   int vid[K_COUNT];
   int zc[K_COUNT];
+  // fused stage pairs: on/off, tile variant and z-chunk per pair kind
+  bool fuse = false;
+  bool pfuse[N_PAIR_KINDS];  // per pair: fused sweep (autotuner's pick) or the two stages
+  int pvid[N_PAIR_KINDS];
+  int pzc[N_PAIR_KINDS];
   // profiling
   bool prof = false;
   struct Ev {
@@ -137,6 +159,27 @@ int grid_for_v(gpe_ctx *c, int vid, int zc, StageArgs &A) {
 }
 
 int grid_for(gpe_ctx *c, int kind, StageArgs &A) { return grid_for_v(c, c->vid[kind], c->zc[kind], A); }
+
+// Grid of a fused pair: tiles step by the (64-2R) x (TY-2R) interiors; the
+// z-chunks are long (each chunk primes 2R extra planes of stage A).
+int grid_for_pair_v(gpe_ctx *c, int vid, int zc, StageArgs &B) {
+  const PairTile t = kPairTiles[vid];
+  const int TXO = 64 - 2 * c->R, TYO = t.nw * t.p - 2 * c->R;
+  B.ntx = (c->P.nx + TXO - 1) / TXO;
+  B.nty = (c->P.ny + TYO - 1) / TYO;
+  const int tiles2d = B.ntx * B.nty;
+  if (zc <= 0) {
+    int nzc = (1024 + tiles2d - 1) / tiles2d;
+    zc = (c->P.nz + nzc - 1) / nzc;
+    if (zc < 64 && tiles2d * ((c->P.nz + 63) / 64) >= 512) zc = 64;
+  }
+  if (zc < 8) zc = 8;
+  if (zc > c->P.nz) zc = c->P.nz;
+  B.zc = zc;
+  B.nzc = (c->P.nz + zc - 1) / zc;
+  B.nblocks = tiles2d * B.nzc;
+  return B.nblocks;
+}
 
 hipEvent_t take_event(gpe_ctx *c) {
   if (!c->pool.empty()) {
@@ -180,6 +223,33 @@ int run_kind(gpe_ctx *c, int kind, StageArgs &A) {
     if (c->pending.size() > 4096) prof_drain(c);
   }
   return GPE_OK;
+}
+
+int run_pair(gpe_ctx *c, int pair, const StageArgs &A, StageArgs &B) {
+  grid_for_pair_v(c, c->pvid[pair], c->pzc[pair], B);
+  if (B.nblocks > c->part_cap) return fail(GPE_EINVAL, "internal: %d workgroups exceed partial buffer", B.nblocks);
+  hipEvent_t ea = nullptr, eb = nullptr;
+  if (c->prof) {
+    ea = take_event(c);
+    eb = take_event(c);
+    if (ea) hipEventRecord(ea, c->stream);
+  }
+  hipError_t e = launch_pair((int)c->esz, c->pvid[pair], pair, c->R, c->comp, A, B, c->stream);
+  if (e != hipSuccess) return fail(GPE_EHIP, "pair %s launch: %s", kind_name(KP0 + pair), hipGetErrorString(e));
+  if (c->prof && ea && eb) {
+    hipEventRecord(eb, c->stream);
+    c->pending.push_back({KP0 + pair, ea, eb});
+    if (c->pending.size() > 4096) prof_drain(c);
+  }
+  return GPE_OK;
+}
+
+// Stages kA, kB = kA+1 of pair `pair`: one fused sweep, or one sweep each.
+int run_two(gpe_ctx *c, int pair, StageArgs &A, StageArgs &B) {
+  static const int first[N_PAIR_KINDS] = {K_RK4_1, K_RK4_3, K_CK_1, K_CK_3, K_CK_5};
+  if (c->fuse && c->pfuse[pair]) return run_pair(c, pair, A, B);
+  int rc = run_kind(c, first[pair], A);
+  return rc ? rc : run_kind(c, first[pair] + 1, B);
 }
 
 StageArgs args_for(gpe_ctx *c) {
@@ -233,6 +303,28 @@ int step_once(gpe_ctx *c, double *hist_slot) {
     A.lo_out = c->lo2;
     if ((rc = run_kind(c, K_RK2_2, A))) return rc;
     if ((rc = reduce_after(c, A, false, false, hist_slot))) return rc;
+    swap_state(c);
+  } else if (sch == GPE_RK4 && c->fuse) {
+    // stages (1,2) and (3,4) pairwise: fused, k1 and k3 never leave the sweep
+    StageArgs A1 = args_for(c), A2 = args_for(c), A3 = args_for(c), A4 = args_for(c);
+    A1.psi = c->psi;
+    A1.kout = c->k[0];
+    A2.psi = c->psi;
+    A2.kin[0] = c->k[0];
+    A2.kout = c->k[1];
+    A2.acc_out = c->acc;
+    if ((rc = run_two(c, P_RK4_12, A1, A2))) return rc;
+    A3.psi = c->psi;
+    A3.kin[0] = c->k[1];
+    A3.kout = c->k[0];
+    A4.psi = c->psi;
+    A4.lo = c->lo;
+    A4.kin[0] = c->k[0];
+    A4.acc_in = c->acc;
+    A4.psi_out = c->psi2;
+    A4.lo_out = c->lo2;
+    if ((rc = run_two(c, P_RK4_34, A3, A4))) return rc;
+    if ((rc = reduce_after(c, A4, false, false, hist_slot))) return rc;
     swap_state(c);
   } else if (sch == GPE_RK4) {
     // k[0]: k1 then k3 (k1 is folded into acc by stage 2), k[1]: k2
@@ -296,7 +388,8 @@ int rkqs(gpe_ctx *c, double *hist_slot) {
   for (int attempt = 0;; ++attempt) {
     c->st.dt_re = (double)dtr;
     c->st.dt_im = (double)dti;
-    for (int s = 1; s <= 6; ++s) {
+    // descriptor of Cash-Karp stage s (kk_s = dt f(psi + sum_m b_sm kk_m))
+    auto ck_args = [&](int s) {
       StageArgs A = args_for(c);
       A.sdtr = sdtr;
       A.sdti = sdti;
@@ -318,13 +411,24 @@ int rkqs(gpe_ctx *c, double *hist_slot) {
           A.write_scale = attempt == 0;
           A.scale_out = c->scale;
         }
-        if ((rc = run_kind(c, K_CK_1 + s - 1, A))) return rc;
       } else {
         A.scale_in = c->scale;
         A.psi_out = c->psi2;
         A.lo_out = c->lo2;
-        if ((rc = run_kind(c, K_CK_6, A))) return rc;
-        if ((rc = reduce_after(c, A, true, false, hist_slot))) return rc;
+      }
+      return A;
+    };
+    if (c->fuse) {
+      for (int s = 1; s <= 5; s += 2) {
+        StageArgs A = ck_args(s), B = ck_args(s + 1);
+        if ((rc = run_two(c, P_CK_12 + s / 2, A, B))) return rc;
+        if (s == 5 && (rc = reduce_after(c, B, true, false, hist_slot))) return rc;
+      }
+    } else {
+      for (int s = 1; s <= 6; ++s) {
+        StageArgs A = ck_args(s);
+        if ((rc = run_kind(c, K_CK_1 + s - 1, A))) return rc;
+        if (s == 6 && (rc = reduce_after(c, A, true, false, hist_slot))) return rc;
       }
     }
     HIPCHK(hipMemcpyAsync(c->hscal, c->dscal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -460,10 +564,23 @@ int gpe_ctx_create(const gpe_params *params, gpe_ctx **out) {
       c->vid[kk] = 1;
       c->zc[kk] = 0;
     }
+    for (int q = 0; q < N_PAIR_KINDS; ++q) {
+      c->pvid[q] = 0;
+      c->pzc[q] = 0;
+      c->pfuse[q] = true;
+    }
+    // the FP32 forms fuse stage pairs by default; FP64 (the parity path)
+    // runs the stages one by one unless asked
+    c->fuse = c->prec != GPE_FP64;
     int cap = 0;
     for (int v = 0; v < N_TILE_VARIANTS; ++v) {
       StageArgs tmp{};
       int nb = grid_for_v(c, v, 8, tmp);
+      if (nb > cap) cap = nb;
+    }
+    for (int v = 0; v < N_PAIR_VARIANTS; ++v) {
+      StageArgs tmp{};
+      int nb = grid_for_pair_v(c, v, 8, tmp);
       if (nb > cap) cap = nb;
     }
     if (cap < PW_BLOCKS) cap = PW_BLOCKS;
@@ -741,8 +858,24 @@ int gpe_solver_step(gpe_ctx *c, const void *var_in, int32_t halo, void *var_out,
 }
 
 
+int gpe_set_fusion(gpe_ctx *c, int32_t enable) {
+  if (!c) return fail(GPE_EINVAL, "NULL context");
+  c->fuse = enable != 0;
+  return GPE_OK;
+}
+
 int gpe_tune_set(gpe_ctx *c, int32_t kind, int32_t variant, int32_t zchunk) {
   if (!c) return fail(GPE_EINVAL, "NULL context");
+  if (kind >= KP0 && kind < KP0 + N_PAIR_KINDS) {
+    const int nv = c->prec == GPE_FP64 ? N_PAIR_VARIANTS_F64 : N_PAIR_VARIANTS;
+    if (variant < -1 || variant >= nv) return fail(GPE_EINVAL, "pair variant out of range");
+    if (zchunk < 0) return fail(GPE_EINVAL, "zchunk < 0");
+    // variant -1: run this pair as its two stages
+    c->pfuse[kind - KP0] = variant >= 0;
+    if (variant >= 0) c->pvid[kind - KP0] = variant;
+    c->pzc[kind - KP0] = zchunk;
+    return GPE_OK;
+  }
   if (kind < 0 || kind >= K_COUNT) return fail(GPE_EINVAL, "kind out of range");
   if (variant < 0 || variant >= N_TILE_VARIANTS) return fail(GPE_EINVAL, "variant out of range");
   if (zchunk < 0) return fail(GPE_EINVAL, "zchunk < 0");
@@ -753,6 +886,15 @@ int gpe_tune_set(gpe_ctx *c, int32_t kind, int32_t variant, int32_t zchunk) {
 
 int gpe_tune_get(gpe_ctx *c, int32_t kind, int32_t *variant, int32_t *zchunk, int32_t *nw, int32_t *rows) {
   if (!c) return fail(GPE_EINVAL, "NULL context");
+  if (kind >= KP0 && kind < KP0 + N_PAIR_KINDS) {
+    const int q = kind - KP0;
+    // variant -1: this pair runs as its two stages
+    if (variant) *variant = (c->fuse && c->pfuse[q]) ? c->pvid[q] : -1;
+    if (zchunk) *zchunk = c->pzc[q];
+    if (nw) *nw = kPairTiles[c->pvid[q]].nw;
+    if (rows) *rows = kPairTiles[c->pvid[q]].p;
+    return GPE_OK;
+  }
   if (kind < 0 || kind >= K_COUNT) return fail(GPE_EINVAL, "kind out of range");
   if (variant) *variant = c->vid[kind];
   if (zchunk) *zchunk = c->zc[kind];
@@ -776,12 +918,80 @@ int gpe_autotune(gpe_ctx *c, int32_t reps) {
     case GPE_RK4: kinds = {K_RK4_1, K_RK4_2, K_RK4_3, K_RK4_4}; break;
     case GPE_RK45: kinds = {K_CK_1, K_CK_2, K_CK_3, K_CK_4, K_CK_5, K_CK_6}; break;
   }
+  std::vector<int> pairs;
+  if (c->fuse && c->P.scheme == GPE_RK4) pairs = {P_RK4_12, P_RK4_34};
+  if (c->fuse && c->P.scheme == GPE_RK45) pairs = {P_CK_12, P_CK_34, P_CK_56};
+  // best time per stage kind and per fused pair; a pair runs fused only if
+  // that beats its two stages one by one
+  float kbest[K_COUNT], pbest[N_PAIR_KINDS];
+  for (float &v : kbest) v = 1e30f;
+  for (float &v : pbest) v = 1e30f;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   const bool prof = c->prof;
   c->prof = false;
   int rc = GPE_OK;
+  // fused pairs: outputs go to work fields every step rewrites before reading
+  for (int pair : pairs) {
+    StageArgs A = args_for(c), B = args_for(c);
+    for (StageArgs *X : {&A, &B}) {
+      X->sdtr = c->st.dt_re;
+      X->sdti = c->st.dt_im;
+      X->psi = c->psi;
+      X->lo = c->lo;
+      X->write_scale = 0;
+      for (int m = 0; m < 5; ++m) {
+        X->kin[m] = c->k[m] ? c->k[m] : c->psi;
+        X->b[m] = 0.1;
+      }
+      X->psi_out = c->psi2;
+      X->lo_out = c->lo2;
+      X->acc_in = c->acc;
+      X->scale_in = c->scale;
+      X->scale_out = c->scale;
+    }
+    if (pair == P_RK4_12) {
+      B.kout = c->k[1];
+      B.acc_out = c->acc;
+    } else if (pair == P_RK4_34) {
+      A.kin[0] = c->k[1];
+    } else if (pair == P_CK_12) {
+      A.kout = c->k[2];
+      B.kout = c->k[3];
+    } else if (pair == P_CK_34) {
+      A.kout = c->k[4];
+      B.kout = c->k[4];
+    }
+    const int zcs[] = {0, 32, 64, 128, 256};
+    const int nv = c->prec == GPE_FP64 ? N_PAIR_VARIANTS_F64 : N_PAIR_VARIANTS;
+    float best = 1e30f;
+    int bv = c->pvid[pair], bz = c->pzc[pair];
+    for (int v = 0; v < nv && rc == GPE_OK; ++v) {
+      for (int zc : zcs) {
+        if (zc > c->P.nz) continue;
+        c->pvid[pair] = v;
+        c->pzc[pair] = zc;
+        if ((rc = run_pair(c, pair, A, B))) break;  // warm
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(GPE_EHIP, "event"); break; }
+        for (int r = 0; r < reps && rc == GPE_OK; ++r) rc = run_pair(c, pair, A, B);
+        if (rc) break;
+        hipEventRecord(e1, c->stream);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+          best = ms;
+          bv = v;
+          bz = zc;
+        }
+      }
+    }
+    c->pvid[pair] = bv;
+    c->pzc[pair] = bz;
+    pbest[pair] = best;
+    if (rc) break;
+  }
   for (int kind : kinds) {
     StageArgs A = args_for(c);
     A.sdtr = c->st.dt_re;
@@ -827,7 +1037,13 @@ int gpe_autotune(gpe_ctx *c, int32_t reps) {
     }
     c->vid[kind] = bv;
     c->zc[kind] = bz;
+    kbest[kind] = best;
     if (rc) break;
+  }
+  {
+    static const int first[N_PAIR_KINDS] = {K_RK4_1, K_RK4_3, K_CK_1, K_CK_3, K_CK_5};
+    // a clear margin, so timing noise does not pick the slower form
+    for (int pair : pairs) c->pfuse[pair] = pbest[pair] < 0.97f * (kbest[first[pair]] + kbest[first[pair] + 1]);
   }
   c->prof = prof;
   hipEventDestroy(e0);
@@ -862,14 +1078,14 @@ int gpe_prof_enable(gpe_ctx *c, int32_t enable) {
 int gpe_prof_query(gpe_ctx *c, int32_t kind, int64_t *launches, double *total_ms, double *bytes_per_launch,
                    char *name) {
   if (!c) return fail(GPE_EINVAL, "NULL context");
-  if (kind < 0 || kind >= K_COUNT) return fail(GPE_EINVAL, "kind out of range");
+  if (kind < 0 || kind >= GPE_PROF_KINDS) return fail(GPE_EINVAL, "kind out of range");
   prof_drain(c);
   if (launches) *launches = c->prof_n[kind];
   if (total_ms) *total_ms = c->prof_ms[kind];
   if (bytes_per_launch) {
     double passes = kind_passes(kind);
     double bytes = passes * (double)c->esz;
-    if (c->comp && (kind == K_EULER || kind == K_RK2_2 || kind == K_RK4_4 || kind == K_CK_6)) bytes += 2.0 * c->esz;
+    if (c->comp && kind_final(kind)) bytes += 2.0 * c->esz;
     *bytes_per_launch = bytes * (double)c->n;
   }
   if (name) {

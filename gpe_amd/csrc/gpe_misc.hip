@@ -51,6 +51,34 @@ hipError_t launch_stage_batched(int prec_bytes, int vid, int kind, int radius, b
   return hipErrorInvalidValue;
 }
 
+hipError_t launch_pair(int prec_bytes, int vid, int pair, int radius, bool comp, const StageArgs &A,
+                       const StageArgs &B, hipStream_t s) {
+  if (prec_bytes == 16) {
+    if (vid == 0) return launch_pair_f64_v0(pair, radius, comp, A, B, s);
+  } else {
+    switch (vid) {
+      case 0: return launch_pair_f32_v0(pair, radius, comp, A, B, s);
+      case 1: return launch_pair_f32_v1(pair, radius, comp, A, B, s);
+      case 2: return launch_pair_f32_v2(pair, radius, comp, A, B, s);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_pair_batched(int prec_bytes, int vid, int pair, int radius, bool comp, const PairBatchLaunch &L,
+                               hipStream_t s) {
+  if (prec_bytes == 16) {
+    if (vid == 0) return launch_pair_batched_f64_v0(pair, radius, comp, L, s);
+  } else {
+    switch (vid) {
+      case 0: return launch_pair_batched_f32_v0(pair, radius, comp, L, s);
+      case 1: return launch_pair_batched_f32_v1(pair, radius, comp, L, s);
+      case 2: return launch_pair_batched_f32_v2(pair, radius, comp, L, s);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_batch_control(InstCtl *ctl, StageArgs *args, int cap, int *active, const int *range,
                                 const double *pn, const double *pe, hipStream_t s) {
   if (cap <= 0) return hipSuccess;

@@ -21,6 +21,23 @@ constexpr Tile kTiles[N_TILE_VARIANTS] = {{4, 4}, {8, 2}, {16, 1}, {4, 2}, {8, 1
 hipError_t launch_stage(int prec_bytes, int vid, int kind, int radius, bool comp, const StageArgs &A,
                         hipStream_t s);
 
+// Fused stage pairs (fused.hpp): pair kinds P_RK4_12 .. P_CK_56 and their
+// tile variants.
+constexpr int N_PAIR_KINDS = 5;
+constexpr int N_PAIR_VARIANTS = 3;
+// waves, rows per wave, planes of prefetch
+struct PairTile {
+  int nw, p, d;
+};
+constexpr PairTile kPairTiles[N_PAIR_VARIANTS] = {{8, 2, 2}, {16, 1, 2}, {16, 1, 4}};
+// Variants compiled for FP64 (the fused FP64 kernels exist to check that
+// fusion changes no bit; the FP64 throughput path runs the stages one by one)
+constexpr int N_PAIR_VARIANTS_F64 = 1;
+// A: stage A's descriptor, B: stage B's (B carries the grid: ntx/nty over
+// the (64-2R) x (TY-2R) interiors, zc, nblocks).
+hipError_t launch_pair(int prec_bytes, int vid, int pair, int radius, bool comp, const StageArgs &A,
+                       const StageArgs &B, hipStream_t s);
+
 // Batched launch over a work table (see stage_kernel_batched).
 struct BatchLaunch {
   const StageArgs *insts;  // this kind's StageArgs of every instance slot
@@ -32,6 +49,16 @@ struct BatchLaunch {
 };
 hipError_t launch_stage_batched(int prec_bytes, int vid, int kind, int radius, bool comp, const BatchLaunch &B,
                                 hipStream_t s);
+struct PairBatchLaunch {
+  const StageArgs *instA, *instB;  // stage A's / B's StageArgs of every instance slot
+  const WorkItem *work;
+  int nwork;
+  const int *active;
+  double *pn, *pe;
+  int grid;
+};
+hipError_t launch_pair_batched(int prec_bytes, int vid, int pair, int radius, bool comp, const PairBatchLaunch &L,
+                               hipStream_t s);
 
 // Per-instance step control after the final stage of a batched iteration.
 hipError_t launch_batch_control(InstCtl *ctl, StageArgs *args, int cap, int *active, const int *range,
@@ -86,5 +113,15 @@ GPE_DECL_BVARIANT(f32, 2)
 GPE_DECL_BVARIANT(f32, 3)
 GPE_DECL_BVARIANT(f32, 4)
 #undef GPE_DECL_BVARIANT
+#define GPE_DECL_PVARIANT(P, V)                                                                              \
+  hipError_t launch_pair_##P##_v##V(int pair, int radius, bool comp, const StageArgs &A, const StageArgs &B,  \
+                                    hipStream_t s);                                                         \
+  hipError_t launch_pair_batched_##P##_v##V(int pair, int radius, bool comp, const PairBatchLaunch &L,       \
+                                            hipStream_t s);
+GPE_DECL_PVARIANT(f64, 0)
+GPE_DECL_PVARIANT(f32, 0)
+GPE_DECL_PVARIANT(f32, 1)
+GPE_DECL_PVARIANT(f32, 2)
+#undef GPE_DECL_PVARIANT
 
 }  // namespace gpe

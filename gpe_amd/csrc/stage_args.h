@@ -25,6 +25,9 @@ enum Kind : int {
   K_COUNT
 };
 
+// Two consecutive stages fused into one sweep (fused.hpp).
+enum Pair : int { P_RK4_12 = 0, P_RK4_34, P_CK_12, P_CK_34, P_CK_56, P_COUNT };
+
 // Division by a real constant as the reference build evaluates it: the
 // divisor is promoted to complex and divided by compiler-rt's logb-scaled
 // quotient, i.e. ((a*cs + b*0)/den, (b*cs - a*0)/den) * 2^-e with

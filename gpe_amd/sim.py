@@ -179,10 +179,16 @@ class Simulation:
         k = _lib.KIND_NAMES.index(kind) if isinstance(kind, str) else int(kind)
         _lib.check(self._L.gpe_tune_set(self._ctx, k, int(variant), int(zchunk)))
 
+    def set_fusion(self, enable=True):
+        """Fuse RK4 / Cash-Karp stages pairwise (bitwise the same fields)."""
+        _lib.check(self._L.gpe_set_fusion(self._ctx, 1 if enable else 0))
+
     def tune_get(self):
-        """{kind name: (variant, zchunk, waves, rows per wave)}"""
+        """{kind name: (variant, zchunk, waves, rows per wave)} for the stage
+        kinds and the fused stage pairs"""
         out = {}
-        for k, nm in enumerate(_lib.KIND_NAMES):
+        for nm in _lib.STAGE_KINDS + _lib.PAIR_KINDS:
+            k = _lib.KIND_NAMES.index(nm)
             v, z, w, r = (ctypes.c_int32() for _ in range(4))
             _lib.check(self._L.gpe_tune_get(self._ctx, k, ctypes.byref(v), ctypes.byref(z), ctypes.byref(w),
                                             ctypes.byref(r)))

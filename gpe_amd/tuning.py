@@ -14,8 +14,13 @@ DEFAULT_PATH = os.environ.get("GPE_TUNING_FILE", os.path.join(os.path.dirname(os
                                                               "tuning.json"))
 
 
-def key(cfg):
-    return "%dx%dx%d/o%d/%s/%s/b%d" % (cfg.nx, cfg.ny, cfg.nz, cfg.order, cfg.precision, cfg.scheme, cfg.bcs)
+def key(cfg, fused=None):
+    """fused: whether the stages run fused pairwise (default: the library's
+    default, on for the FP32 forms)."""
+    if fused is None:
+        fused = cfg.precision != "fp64"
+    return "%dx%dx%d/o%d/%s/%s/b%d%s" % (cfg.nx, cfg.ny, cfg.nz, cfg.order, cfg.precision, cfg.scheme, cfg.bcs,
+                                         "/fused" if fused else "")
 
 
 def load(path=DEFAULT_PATH):

@@ -127,17 +127,24 @@ int gpe_sim_renormalise(gpe_ctx *ctx, double nn, double norm);
 /* f(psi) of the resident field into a host buffer (get_rhs, solve.f90:382). */
 int gpe_sim_rhs(gpe_ctx *ctx, void *out, int32_t host_type);
 
-/* Launch configuration of the stage kernels.  Each stage kind (0 ..
- * GPE_PROF_KINDS-1, see gpe_prof_query for names) runs with one of several
- * compiled tile variants (waves x rows per wave) and a z-chunk length
- * (0 = automatic).  No reference counterpart: the reference has no device
- * kernels to configure. */
+/* Launch configuration of the stage kernels.  Each stage kind (0 .. 14)
+ * and each fused stage pair (16 .. 20; see gpe_prof_query for names) runs
+ * with one of several compiled tile variants (waves x rows per wave) and a
+ * z-chunk length (0 = automatic); a pair with variant -1 runs as its two
+ * stages.  No reference counterpart: the reference has no device kernels to
+ * configure. */
 int gpe_tune_set(gpe_ctx *ctx, int32_t kind, int32_t variant, int32_t zchunk);
+/* Fuse consecutive RK4 / Cash-Karp stages pairwise into one sweep each
+ * (RK4: 7 field passes per step instead of 13; Cash-Karp: 14 per attempt
+ * instead of 27).  Results are bitwise those of the stage-by-stage sweeps.
+ * Default: on for GPE_FP32 / GPE_FP32C, off for GPE_FP64. */
+int gpe_set_fusion(gpe_ctx *ctx, int32_t enable);
 int gpe_tune_get(gpe_ctx *ctx, int32_t kind, int32_t *variant, int32_t *zchunk, int32_t *waves,
                  int32_t *rows_per_wave);
-/* Time every variant and several z-chunks of each stage kind of the
- * context's scheme on the resident field (reps launches each) and keep the
- * fastest.  The state is unchanged. */
+/* Time every variant and several z-chunks of each stage kind (and, with
+ * fusion on, of each stage pair) of the context's scheme on the resident
+ * field (reps launches each) and keep the fastest; a pair runs fused only
+ * where that beats its two stages.  The state is unchanged. */
 int gpe_autotune(gpe_ctx *ctx, int32_t reps);
 
 /* Kernel timing with HIP events recorded on the context's stream around
@@ -145,7 +152,7 @@ int gpe_autotune(gpe_ctx *ctx, int32_t reps);
 int gpe_prof_enable(gpe_ctx *ctx, int32_t enable);
 /* For kernel kind `kind` (0..GPE_PROF_KINDS-1): launches, total ms and
  * algorithmic HBM bytes per launch; the name is written to name[64]. */
-#define GPE_PROF_KINDS 16
+#define GPE_PROF_KINDS 24
 int gpe_prof_query(gpe_ctx *ctx, int32_t kind, int64_t *launches, double *total_ms,
                    double *bytes_per_launch, char *name);
 
