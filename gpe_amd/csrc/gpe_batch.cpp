@@ -94,8 +94,16 @@ struct Group {
 
 }  // namespace
 
+constexpr int kAuxStreams = 3;
+constexpr int kMaxGroups = 16;  // 2 precisions x 2 orders x 4 schemes
+
 struct gpe_batch {
   hipStream_t stream = nullptr;
+  // captured iterations run the groups concurrently: forked onto these
+  // streams and joined before the control kernel
+  hipStream_t aux[kAuxStreams] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_done[kMaxGroups] = {};
   int cap = 0;                 // instance slots in the device tables
   StageArgs *d_args = nullptr; // [K_COUNT][cap]
   InstCtl *d_ctl = nullptr;    // [cap]
@@ -372,9 +380,19 @@ int rebuild(gpe_batch *b) {
 }
 
 // One iteration: every group's stages, then the per-instance control.
-int enqueue_iteration(gpe_batch *b) {
+int enqueue_iteration(gpe_batch *b, bool fork) {
+  int ngroups = 0;
+  for (Group &G : b->groups) ngroups += !G.slots.empty();
+  fork = fork && ngroups > 1;
+  if (fork) GPE_HIPCHK(hipEventRecord(b->ev_fork, b->stream));
+  int j = 0;
   for (Group &G : b->groups) {
     if (G.slots.empty()) continue;
+    hipStream_t s = b->stream;
+    if (fork) {
+      s = b->aux[j % kAuxStreams];
+      GPE_HIPCHK(hipStreamWaitEvent(s, b->ev_fork, 0));
+    }
     for (int k = G.k0; k < G.k0 + G.nk; ++k) {
       const KindTable &T = G.tab[k];
       if (!T.n) continue;
@@ -387,11 +405,14 @@ int enqueue_iteration(gpe_batch *b) {
       L.pn = fin ? b->d_pn : nullptr;
       L.pe = fin ? b->d_pe : nullptr;
       L.grid = T.grid;
-      hipError_t e = launch_stage_batched(G.prec == GPE_FP64 ? 16 : 8, G.vid[k], k, G.R, G.prec == GPE_FP32C, L,
-                                          b->stream);
+      hipError_t e = launch_stage_batched(G.prec == GPE_FP64 ? 16 : 8, G.vid[k], k, G.R, G.prec == GPE_FP32C, L, s);
       if (e != hipSuccess) return fail(GPE_EHIP, "batched stage launch: %s", hipGetErrorString(e));
     }
+    if (fork) GPE_HIPCHK(hipEventRecord(b->ev_done[j], s));
+    ++j;
   }
+  if (fork)
+    for (int q = 0; q < j; ++q) GPE_HIPCHK(hipStreamWaitEvent(b->stream, b->ev_done[q], 0));
   hipError_t e = launch_batch_control(b->d_ctl, b->d_args, b->cap, b->d_active, b->d_range, b->d_pn, b->d_pe,
                                       b->stream);
   if (e != hipSuccess) return fail(GPE_EHIP, "batch control launch: %s", hipGetErrorString(e));
@@ -400,7 +421,7 @@ int enqueue_iteration(gpe_batch *b) {
 
 int capture(gpe_batch *b) {
   GPE_HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_iteration(b);
+  int rc = enqueue_iteration(b, true);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   if (rc) {
@@ -466,6 +487,13 @@ int gpe_batch_create(int64_t hbm_budget, gpe_batch **out) {
     delete b;
     return fail(GPE_EHIP, "hipStreamCreate failed");
   }
+  bool ok = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+  for (int q = 0; q < kAuxStreams && ok; ++q) ok = hipStreamCreateWithFlags(&b->aux[q], hipStreamNonBlocking) == hipSuccess;
+  for (int q = 0; q < kMaxGroups && ok; ++q) ok = hipEventCreateWithFlags(&b->ev_done[q], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    gpe_batch_destroy(b);
+    return fail(GPE_EHIP, "batch streams/events");
+  }
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
   // default: 90% of what is free now
@@ -493,6 +521,11 @@ void gpe_batch_destroy(gpe_batch *b) {
   void *ptrs[] = {b->d_args, b->d_ctl, b->d_active, b->d_range, b->d_pn, b->d_pe};
   for (void *p : ptrs)
     if (p) hipFree(p);
+  for (hipStream_t q : b->aux)
+    if (q) hipStreamDestroy(q);
+  for (hipEvent_t ev : b->ev_done)
+    if (ev) hipEventDestroy(ev);
+  if (b->ev_fork) hipEventDestroy(b->ev_fork);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
 }
@@ -631,7 +664,7 @@ int gpe_batch_run(gpe_batch *b, int32_t iterations) {
   for (int i = 0; i < iterations; ++i) {
     if (b->use_graph) {
       GPE_HIPCHK(hipGraphLaunch(b->exec, b->stream));
-    } else if ((rc = enqueue_iteration(b))) {
+    } else if ((rc = enqueue_iteration(b, false))) {
       return rc;
     }
   }

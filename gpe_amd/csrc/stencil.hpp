@@ -24,8 +24,10 @@
 // expression in the reference's order with its complex semantics (complex
 // division by a real in compiler-rt's logb-scaled form, abs() as hypot), so
 // they reproduce the reference arithmetic operation by operation.  FP32
-// kernels use reciprocals and FMA; their final psi update and all
-// reductions are FP64 (with an FP32 hi/lo pair when compensated).
+// kernels multiply by reciprocals of the constant divisors and are also
+// built without contraction (one rounding per operation, so every
+// instantiation of a stage gives the same bits); their final psi update and
+// all reductions are FP64 (with an FP32 hi/lo pair when compensated).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -425,7 +427,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
         if (m >= R) make_F(r, F[m - R][p]);
       }
     }
-#ifndef GPE_GHOST_LATE
     if (m >= R) {
 #pragma unroll
       for (int i = 0; i < HPT; ++i) {
@@ -438,49 +439,49 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
         }
       }
     }
-#endif
   }
 
-  // Loads issued for iteration k: interior and ghosts of plane k+R,
-  // output-time fields of plane k.
-  Raw rn[P];
-  Raw hn[HPT];
+  // Loads in flight: interior and ghosts of plane k+R (D planes ahead for
+  // the stages that read at most two fields per site, whose single plane of
+  // prefetch would leave HBM idle between planes), output-time fields of
+  // plane k.
+  constexpr int D = (NIN + (KI::has_acc ? 1 : 0) + 1 <= 2) ? 2 : 1;
+  Raw rn[D][P];
+  Raw hn[D][HPT];
   cx<T> on[P][NOA];
-  auto issue = [&](int k) {
+  auto issue_raw = [&](int k, Raw *rb, Raw *hb) {
     const size_t zi = (size_t)bcmap(k + R, nz, per) * plane;
-    const size_t zk = (size_t)k * plane;
 #pragma unroll
-    for (int p = 0; p < P; ++p) load_raw(zi + roff[p], rn[p], true);
+    for (int p = 0; p < P; ++p) load_raw(zi + roff[p], rb[p], true);
 #pragma unroll
     for (int i = 0; i < HPT; ++i)
-#ifdef GPE_GHOST_LATE
-      if (hl[i] >= 0) load_raw(zk + hoff[i], hn[i], false);
-#else
-      if (hl[i] >= 0) load_raw(zi + hoff[i], hn[i], false);
-#endif
+      if (hl[i] >= 0) load_raw(zi + hoff[i], hb[i], false);
+  };
+  auto issue_out = [&](int k) {
     if constexpr (NO > 0) {
 #pragma unroll
-      for (int p = 0; p < P; ++p) load_out(zk + roff[p], on[p]);
+      for (int p = 0; p < P; ++p) load_out((size_t)k * plane + roff[p], on[p]);
     }
   };
-  issue(z0);
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    if (z0 + d < z1) issue_raw(z0 + d, rn[d], hn[d]);
+  issue_out(z0);
 
   double acc_norm = 0.0, acc_emax = 0.0, acc_en = 0.0;
 
-  for (int k = z0; k < z1; ++k) {
+  // one plane of the march; rb/hb hold plane k+R and are refilled with
+  // plane k+D+R
+  auto step = [&](const int k, Raw *rb, Raw *hb) {
     V *Ld = lds[(k - z0) & 1];
     // consume what was prefetched for plane k
 #pragma unroll
     for (int p = 0; p < P; ++p) {
-      q[2 * R][p] = make_u(rn[p]);
-      if constexpr (NF > 0) make_F(rn[p], F[R][p]);
+      q[2 * R][p] = make_u(rb[p]);
+      if constexpr (NF > 0) make_F(rb[p], F[R][p]);
     }
 #pragma unroll
-#ifdef GPE_GHOST_LATE
-    for (int i = 0; i < HPT; ++i) G[0][i] = hl[i] >= 0 ? make_u(hn[i]) : mk((T)0, (T)0);
-#else
-    for (int i = 0; i < HPT; ++i) G[R][i] = hl[i] >= 0 ? make_u(hn[i]) : mk((T)0, (T)0);
-#endif
+    for (int i = 0; i < HPT; ++i) G[R][i] = hl[i] >= 0 ? make_u(hb[i]) : mk((T)0, (T)0);
     cx<T> oc[P][NOA];
     if constexpr (NO > 0) {
 #pragma unroll
@@ -488,8 +489,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
 #pragma unroll
         for (int o = 0; o < NO; ++o) oc[p][o] = on[p][o];
     }
-    // prefetch plane k+1 while plane k is computed
-    if (k + 1 < z1) issue(k + 1);
+    // prefetch while plane k is computed
+    if (k + D < z1) issue_raw(k + D, rb, hb);
+    if (k + 1 < z1) issue_out(k + 1);
     // stage input of plane k into LDS: ghosts and interior
 #pragma unroll
     for (int i = 0; i < HPT; ++i) {
@@ -663,6 +665,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
         for (int p = 0; p < P; ++p)
 #pragma unroll
           for (int f = 0; f < NF; ++f) F[m][p][f] = F[m + 1][p][f];
+    }
+  };
+  // unrolled by D so that every prefetch buffer has a fixed register home
+  for (int k = z0; k < z1; k += D) {
+    step(k, rn[0], hn[0]);
+    if constexpr (D > 1) {
+      if (k + 1 < z1) step(k + 1, rn[1], hn[1]);
     }
   }
 

@@ -102,7 +102,7 @@ def test_tile_variants_fp64(gpu, oracle, variant, scheme):
     say("variant", variant, scheme)
     want, _ = oracle.run(cfg, oracle.make_state(cfg.dt), psi, n, NT)
     with Simulation(cfg, psi) as sim:
-        for k in _lib.KIND_NAMES:
+        for k in _lib.STAGE_KINDS:
             sim.tune_set(k, variant, 8)
         sim.advance(n)
         got = sim.download()
