@@ -117,7 +117,8 @@ def lib():
         "gpe_batch_run": ([vp, i32], ctypes.c_int),
         "gpe_batch_sync": ([vp], ctypes.c_int),
         "gpe_batch_poll": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i32), i32, ctypes.POINTER(i32)], ctypes.c_int),
-        "gpe_batch_query": ([vp, i32, S, ctypes.POINTER(i64), ctypes.POINTER(i32), dp], ctypes.c_int),
+        "gpe_batch_query": ([vp, i32, S, ctypes.POINTER(i64), ctypes.POINTER(i32), dp, ctypes.POINTER(i32)],
+                            ctypes.c_int),
         "gpe_batch_download": ([vp, i32, vp, i32], ctypes.c_int),
         "gpe_batch_stats": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i64),
                              ctypes.POINTER(i64)], ctypes.c_int),

@@ -13,6 +13,9 @@ parameters.in:23-29); an instance's results equal a `Simulation` run of the
 same configuration.
 """
 import ctypes
+import dataclasses
+import json
+import os
 
 import numpy as np
 
@@ -36,6 +39,7 @@ class Batch:
         self._b = ctypes.c_void_p()
         _lib.check(self._L.gpe_batch_create(int(hbm_budget), ctypes.byref(self._b)))
         self.cfgs = {}
+        self.targets = {}
 
     def close(self):
         if self._b:
@@ -78,11 +82,13 @@ class Batch:
             return None
         _lib.check(rc)
         self.cfgs[i.value] = cfg
+        self.targets[i.value] = int(nsteps)
         return i.value
 
     def remove(self, iid):
         _lib.check(self._L.gpe_batch_remove(self._b, int(iid)))
         self.cfgs.pop(iid, None)
+        self.targets.pop(iid, None)
 
     # -- running -----------------------------------------------------------
     def run(self, iterations=1):
@@ -118,10 +124,12 @@ class Batch:
         n = ctypes.c_int64()
         s = ctypes.c_int32()
         nrm = ctypes.c_double()
+        pend = ctypes.c_int32()
         _lib.check(self._L.gpe_batch_query(self._b, int(iid), ctypes.byref(st), ctypes.byref(n), ctypes.byref(s),
-                                           ctypes.byref(nrm)))
+                                           ctypes.byref(nrm), ctypes.byref(pend)))
         return {"dt": complex(st.dt_re, st.dt_im), "t": st.t, "im_t": st.im_t, "errmax": st.errmax,
-                "n_rejects": st.n_rejects, "steps": n.value, "status": s.value, "norm": nrm.value}
+                "n_rejects": st.n_rejects, "steps": n.value, "status": s.value, "norm": nrm.value,
+                "pending_rejects": pend.value}
 
     def download(self, iid, dtype=np.complex128):
         cfg = self.cfgs[iid]
@@ -157,3 +165,48 @@ class Batch:
         """Pick tile variant and z-chunk per stage kind by timing the
         batched launches on the current instances; commits no step."""
         _lib.check(self._L.gpe_batch_autotune(self._b, int(reps)))
+
+    # -- checkpoint / restart ------------------------------------------------
+    def checkpoint(self, dirpath, ids=None, max_iterations=1000):
+        """Write a restart point of the given (default: all running)
+        instances: each field in the reference's end_state format
+        (io.f90:843-853; complex128, the compensated form as hi + lo) plus a
+        manifest with its configuration and step counts.  rk45 instances in
+        the middle of a step (after a rejected attempt) are first run to the
+        end of that step, so every saved state is a step boundary."""
+        from .state_io import write_state
+        ids = list(self.cfgs) if ids is None else list(ids)
+        for _ in range(max_iterations):
+            q = {i: self.query(i) for i in ids}
+            if all(v["pending_rejects"] == 0 or v["status"] != 0 for v in q.values()):
+                break
+            self.run(1)
+        else:
+            raise RuntimeError("rk45 instances did not reach a step boundary")
+        os.makedirs(dirpath, exist_ok=True)
+        manifest = {}
+        for i in ids:
+            v = q[i]
+            if v["status"] != 0:
+                continue
+            fn = "end_state_%d.dat" % i
+            write_state(os.path.join(dirpath, fn), self.download(i), p=v["steps"], t=v["t"], im_t=v["im_t"],
+                        dt=v["dt"])
+            manifest[str(i)] = {"file": fn, "cfg": dataclasses.asdict(self.cfgs[i]), "steps_done": v["steps"],
+                                "steps_target": self.targets.get(i, v["steps"])}
+        with open(os.path.join(dirpath, "manifest.json"), "w") as fh:
+            json.dump(manifest, fh, indent=1)
+        return manifest
+
+    def restore(self, dirpath):
+        """Re-admit the instances of a checkpoint for their remaining steps;
+        returns {old id: new id} (None where the budget is full)."""
+        from .state_io import read_state
+        with open(os.path.join(dirpath, "manifest.json")) as fh:
+            manifest = json.load(fh)
+        out = {}
+        for old, ent in manifest.items():
+            cfg = RunConfig(**ent["cfg"])
+            psi, p, t, im_t, dt = read_state(os.path.join(dirpath, ent["file"]))
+            out[int(old)] = self.add(cfg, psi, ent["steps_target"] - ent["steps_done"], dt=dt, t=t, im_t=im_t)
+        return out

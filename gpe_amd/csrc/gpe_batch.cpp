@@ -697,7 +697,7 @@ int gpe_batch_poll(gpe_batch *b, int32_t *ids, int32_t *status, int32_t cap, int
 }
 
 int gpe_batch_query(gpe_batch *b, int32_t id, gpe_step_state *st, int64_t *steps_done, int32_t *status,
-                    double *norm) {
+                    double *norm, int32_t *pending_rejects) {
   if (!b) return fail(GPE_EINVAL, "batch is NULL");
   const int s = find(b, id);
   if (s < 0) return fail(GPE_EINVAL, "no instance %d in the batch", id);
@@ -716,6 +716,7 @@ int gpe_batch_query(gpe_batch *b, int32_t id, gpe_step_state *st, int64_t *steps
   if (steps_done) *steps_done = C.steps_done;
   if (status) *status = C.status;
   if (norm) *norm = C.norm;
+  if (pending_rejects) *pending_rejects = C.n_rejects;
   return GPE_OK;
 }
 

@@ -143,6 +143,19 @@ class Simulation:
     def mass(self):
         return self._scalar(self._L.gpe_sim_mass)
 
+    def save_end_state(self, path):
+        """The field and step state in the reference's end_state format
+        (io.f90:843-853)."""
+        from .state_io import write_state
+        st = self.state()
+        write_state(path, self.download(), p=st["p"], t=st["t"], im_t=st["im_t"], dt=st["dt"])
+
+    def save_dens(self, proc_dir):
+        """3-D dump in the format of idl_surface (io.f90:480-520)."""
+        from .state_io import write_dens
+        st = self.state()
+        return write_dens(proc_dir, self.download(), self.cfg, st["p"], st["t"], st["im_t"])
+
     def linelength(self):
         """Total vortex-line length (linelength, variables.f90:896-1162)."""
         return self._scalar(self._L.gpe_sim_linelength)

@@ -201,9 +201,11 @@ int gpe_batch_sync(gpe_batch *b);
  * -1 NaN in the rk45 error norm, -2 more than 1000 rejections); n <= cap. */
 int gpe_batch_poll(gpe_batch *b, int32_t *ids, int32_t *status, int32_t cap, int32_t *n);
 /* State of one instance: st as solver() leaves it (p = steps done),
- * status (0 running), and get_norm of its current field. */
+ * status (0 running), get_norm of its current field, and the rk45
+ * attempts rejected in the step in progress (0 at a step boundary, where
+ * st, the field and the step count are a complete restart point). */
 int gpe_batch_query(gpe_batch *b, int32_t id, gpe_step_state *st, int64_t *steps_done, int32_t *status,
-                    double *norm);
+                    double *norm, int32_t *pending_rejects);
 int gpe_batch_download(gpe_batch *b, int32_t id, void *psi, int32_t host_type);
 int gpe_batch_stats(gpe_batch *b, int32_t *instances, int32_t *running, int64_t *bytes_in_use,
                     int64_t *budget);
