@@ -17,6 +17,7 @@ simulation run alone (see tests/test_serve.py).
 """
 import argparse
 import json
+import os
 import sys
 import time
 from collections import deque
@@ -26,6 +27,7 @@ import numpy as np
 
 from . import batch as _batch
 from .batch import Batch
+from .workloads import Request
 
 
 class _Ready:
@@ -65,13 +67,17 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def serve(self, requests, duration=None, drain=True, clock=None, log=None):
+    def serve(self, requests, duration=None, drain=True, clock=None, log=None, checkpoint_dir=None,
+              resume_dir=None):
         """Serve `requests` (sorted by .arrival, seconds from the start).
 
         duration: stop admitting at this time (None: admit all).  drain: keep
         running after the last admission until every admitted request is
         done.  clock: callable giving seconds since the start (default: wall
-        clock).  Returns the metrics dict."""
+        clock).  checkpoint_dir: when the window ends with requests still
+        running, save them there (fields in the end_state format, at step
+        boundaries) instead of dropping them.  resume_dir: re-admit the
+        requests of such a checkpoint first.  Returns the metrics dict."""
         t0 = time.perf_counter()
         if clock is None:
             clock = lambda: time.perf_counter() - t0  # noqa: E731
@@ -79,6 +85,15 @@ class Server:
         queue = deque()
         futures = {}
         running = {}      # batch id -> (request, admit time, sites)
+        if resume_dir is not None:
+            with open(os.path.join(resume_dir, "requests.json")) as fh:
+                meta = json.load(fh)
+            for old, new in self.batch.restore(resume_dir).items():
+                if new is None:
+                    raise RuntimeError("a checkpointed request does not fit the HBM budget")
+                m = meta[str(old)]
+                r = Request(m["rid"], m["arrival"], self.batch.cfgs[new], m["steps"], m["ic_kind"], m["seed"])
+                running[new] = (r, clock(), r.cfg.nx * r.cfg.ny * r.cfg.nz)
         done = []         # (request, arrival, admitted, finished, status, steps)
         resident_sites = 0
         chunk = 4
@@ -152,6 +167,12 @@ class Server:
         end = clock()
         for f in futures.values():
             f.cancel()
+        if checkpoint_dir is not None and running:
+            self.batch.checkpoint(checkpoint_dir, ids=list(running))
+            meta = {str(i): {"rid": r.rid, "arrival": r.arrival, "steps": r.steps, "ic_kind": r.ic_kind,
+                             "seed": r.seed} for i, (r, _, _) in running.items()}
+            with open(os.path.join(checkpoint_dir, "requests.json"), "w") as fh:
+                json.dump(meta, fh)
         # work done on requests still running counts toward throughput
         su_partial = 0.0
         for iid, (r, _, sites) in running.items():

@@ -98,3 +98,32 @@ def test_served_results_equal_alone(budget_fields):
         assert info["steps"] == r.steps
         assert np.array_equal(g, want), r
         assert info["t"] == q["t"]
+
+
+@pytest.mark.gpu
+def test_checkpoint_and_resume_equal_alone(tmp_path):
+    """A serving window that ends with requests running checkpoints them;
+    a new server resumes them; every result still equals its run alone."""
+    from gpe_amd.batch import Batch
+    from gpe_amd.serve import Server
+    reqs = trace()
+    ck = str(tmp_path / "ck")
+    say("serve checkpoint / resume")
+    got = {}
+
+    def deliver(r, psi, info):
+        got[r.rid] = psi
+
+    with Server(target_chunk_s=0.001, deliver=deliver) as s:
+        m1 = s.serve(reqs, duration=1.2, drain=False, clock=FakeClock(), checkpoint_dir=ck)
+    assert m1["unfinished"] > 0
+    with Server(target_chunk_s=0.001, deliver=deliver) as s:
+        m2 = s.serve([], clock=FakeClock(), resume_dir=ck)
+    arrived = [r for r in reqs if r.arrival < 1.2]
+    assert sorted(got) == sorted(r.rid for r in arrived), (sorted(got), m1, m2)
+    for r in arrived:
+        with Batch() as b:
+            i = b.add(r.cfg, r.initial_state(), r.steps)
+            b.run_to_completion(chunk=5)
+            want = b.download(i)
+        assert np.array_equal(got[r.rid], want), r
