@@ -15,13 +15,16 @@ With periodic walls int Re(-lap(psi) psi*) = 2K, so the reference functional
 E_ref = 2K + P (P = (1/2) int |psi|^4) gives H = (E_ref + P)/2 - mu N; P and N
 are summed on the host from the downloaded field.
 
-Each step of the FP32C run starts from the FP64 run's time and proposed dt
-(its own step control still accepts or rejects), so both fields are
-compared at the same times; the number of steps where the two controls
-ended at different times is reported (0 means every accepted dt matched).
-The random-phase tangle is chaotic, so a control run measures how much any
-perturbation grows: FP64 again, from the initial field rounded to
-complex64 (a relative change of ~3e-8), on the same time sequence.
+The FP32C run takes exactly the FP64 run's accepted time steps (its error
+tolerance is set so high that every first attempt is accepted, and each
+step's dt is set to the FP64 step's), so the two fields are compared at the
+same times and the difference is the storage/arithmetic precision alone.
+(With its own step control the FP32C run accepts or rejects differently on
+a few percent of steps, which shifts its clock; that difference is a time
+offset, not an accuracy loss.)  A control run measures how much any
+perturbation of this chaotic tangle grows: FP64 again, from the initial
+field rounded to complex64 (a relative change of ~3e-8), on the same time
+sequence.
 
 A second report covers the 64^3 vortex ring (dx = 0.4, reflective, 2nd
 order, RK4 dt = 0.01, 1000 steps): FP32C against FP64 after 1000 RK4 steps.
@@ -60,24 +63,26 @@ def run(n=512, steps=1000, seed=20261202, log=None):
     psi0 = ic.random_phase(workload("fp64", n), seed)
     out = {"grid": n, "steps": steps, "seed": seed}
     c64, c32 = workload("fp64", n), workload("fp32c", n)
-    with Simulation(c64, psi0) as s64, Simulation(c32, psi0) as s32, \
-            Simulation(c64, psi0.astype(np.complex64).astype(np.complex128)) as sct:
+    follow = 1e30  # tolerance of the runs that follow the FP64 run's steps
+    with Simulation(c64, psi0) as s64, Simulation(c32.with_(eps=follow), psi0) as s32, \
+            Simulation(c64.with_(eps=follow), psi0.astype(np.complex64).astype(np.complex128)) as sct:
         del psi0
         _, o64 = _observables(s64, c64)
         _, o32 = _observables(s32, c32)
         start = {"fp64": o64, "fp32c": o32}
-        diverged = 0
+        rejected = 0
         t0 = time.perf_counter()
         for i in range(steps):
-            st = s64.state()
-            for s in (s32, sct):
-                s.set_state(st["dt"], t=st["t"], im_t=st["im_t"], p=i)
-                s.advance(1)
+            before = s64.state()
             s64.advance(1)
-            if s32.state()["t"] != s64.state()["t"]:
-                diverged += 1
+            after = s64.state()
+            rejected += after["n_rejects"]
+            dt = complex(after["t"] - before["t"], 0.0) if c64.real_time else complex(0.0, before["im_t"] - after["im_t"])
+            for s in (s32, sct):
+                s.set_state(dt, t=before["t"], im_t=before["im_t"], p=i)
+                s.advance(1)
             if (i + 1) % 100 == 0:
-                log("step %d, t = %.6f, %.1f s" % (i + 1, s64.state()["t"], time.perf_counter() - t0))
+                log("step %d, t = %.6f, %.1f s" % (i + 1, after["t"], time.perf_counter() - t0))
         p64, e64 = _observables(s64, c64)
         p32, e32 = _observables(s32, c32)
         pct = sct.download()
@@ -90,7 +95,7 @@ def run(n=512, steps=1000, seed=20261202, log=None):
                      "energy_ref_drift": (e["energy_ref"] - o["energy_ref"]) / abs(o["energy_ref"]),
                      "hamiltonian_drift": (e["hamiltonian"] - o["hamiltonian"]) / abs(o["hamiltonian"]),
                      "norm_drift": (e["norm"] - o["norm"]) / o["norm"]}
-    out["steps_with_different_dt"] = diverged
+    out["fp64_rejected_attempts"] = rejected
     out["psi_rel_l2"] = float(np.linalg.norm((p32 - p64).ravel()) / np.linalg.norm(p64.ravel()))
     out["linelength_rel_diff"] = (e32["linelength"] - e64["linelength"]) / e64["linelength"]
     out["hamiltonian_rel_diff"] = (e32["hamiltonian"] - e64["hamiltonian"]) / abs(e64["hamiltonian"])

@@ -194,8 +194,11 @@ class Server:
         def pct(a, q):
             return float(np.percentile(a, q)) if a.size else None
 
+        failures = [{"rid": d[0].rid, "status": d[4], "scheme": d[0].cfg.scheme, "precision": d[0].cfg.precision,
+                     "n": d[0].cfg.nx, "order": d[0].cfg.order, "bcs": d[0].cfg.bcs, "ic": d[0].ic_kind,
+                     "steps_done": d[5]} for d in done if d[4] != 1][:20]
         return {
-            "completed": len(done), "ok": ok, "failed": len(done) - ok,
+            "completed": len(done), "ok": ok, "failed": len(done) - ok, "failures": failures,
             "unfinished": len(unfinished), "not_arrived": len(pending),
             "elapsed_s": end, "solves_per_s": ok / end if end > 0 else 0.0,
             "site_updates_per_s": su_done / end if end > 0 else 0.0,
