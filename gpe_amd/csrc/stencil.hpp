@@ -256,6 +256,11 @@ struct OutInfo {
   static constexpr int no = (KIND == K_CK_6 ? 1 : 0) + ((COMP && KindInfo<KIND>::final_) ? 1 : 0);
 };
 
+// Stage kinds reading at most this many fields per site prefetch two planes.
+#ifndef GPE_PREFETCH2_MAX
+#define GPE_PREFETCH2_MAX 2
+#endif
+
 // Workgroup barrier that waits only for this wave's LDS traffic: the global
 // loads issued ahead for the next plane stay in flight across it.
 #ifdef GPE_PLAIN_BARRIER
@@ -441,11 +446,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
     }
   }
 
-  // Loads in flight: interior and ghosts of plane k+R (D planes ahead for
-  // the stages that read at most two fields per site, whose single plane of
+  // Loads in flight: interior and ghosts of plane k+R (two planes ahead
+  // for the stages that read few fields per site, whose single plane of
   // prefetch would leave HBM idle between planes), output-time fields of
   // plane k.
-  constexpr int D = (NIN + (KI::has_acc ? 1 : 0) + 1 <= 2) ? 2 : 1;
+  constexpr int NREAD = NIN + (KI::has_acc ? 1 : 0) + 1;
+  constexpr int D = NREAD <= GPE_PREFETCH2_MAX ? 2 : 1;
   Raw rn[D][P];
   Raw hn[D][HPT];
   cx<T> on[P][NOA];
