@@ -110,6 +110,7 @@ __global__ void __launch_bounds__(64) batch_control(InstCtl *__restrict__ ctl, S
     A.hi = 0.5 * C.dt_im;
     A.sdtr = C.sdt_re;
     A.sdti = C.sdt_im;
+    A.inv_absdt = 1.0 / hypot(C.dt_re, C.dt_im);
     A.write_scale = C.first_attempt;
     A.psi = C.buf[C.cur];
     A.psi_out = C.buf[C.cur ^ 1];

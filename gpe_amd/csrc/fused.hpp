@@ -202,6 +202,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
         F[0] = r.ps;
         F[1] = pd;
         F[2] = e;
+        F[3] = scale_from_kk1(B, r.ps, r.kk[0]);
       } else {
         cx<T> s = r.ps;
 #pragma unroll
@@ -228,7 +229,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
   };
   auto load_out = [&](size_t s, cx<T> *o) {
     int i = 0;
-    if constexpr (KB == K_CK_6) {
+    if constexpr (KB == K_CK_6 && !COMP) {
       if (B.scale_real)
         o[i++] = mk(static_cast<const T *>(B.scale_in)[s], (T)0);
       else
@@ -384,7 +385,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
         if (out_x && out_y[p] && ka >= z0 && ka < z1) {
           const size_t s = (size_t)ka * plane + (size_t)gj[p] * nx + gi;
           if constexpr (PI::store_a) stc<T>(A.kout, s, kA);
-          if constexpr (KA == K_CK_1) {
+          if constexpr (KA == K_CK_1 && !COMP) {
             if (A.write_scale) {
               // psi_scal = abs(in) + dt*abs(f) + 1e-30 (solve.f90:176-178)
               T af = cabs_(f), au = cabs_(c0);
@@ -501,8 +502,8 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
           if constexpr (KB != K_CK_6) {
             stc<T>(B.kout, s, kk);
           } else {
-            const cx<T> sc = oc[p][0];
-            const cx<T> err = F[NF - 1] + rmul((T)B.dc[6], kk);
+            const cx<T> sc = COMP ? F[NF - 1] : oc[p][0];
+            const cx<T> err = F[COMP ? 2 : 1] + rmul((T)B.dc[6], kk);
             cx<double> np;
             if constexpr (IS_DBL) {
               np = F[0] + rmul((T)B.c[6], kk);
@@ -512,7 +513,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
             } else {
               if constexpr (COMP) {
                 const cx<double> dl = todbl(F[1]) + rmul(B.c[6], todbl(kk));
-                np = (todbl(F[0]) + todbl(oc[p][1])) + dl;
+                np = (todbl(F[0]) + todbl(oc[p][0])) + dl;
                 const cx<T> hi = mk((T)np.re, (T)np.im);
                 stc<T>(B.psi_out, s, hi);
                 stc<T>(B.lo_out, s, mk((T)(np.re - (double)hi.re), (T)(np.im - (double)hi.im)));

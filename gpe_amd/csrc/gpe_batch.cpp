@@ -60,6 +60,8 @@ size_t footprint(const gpe_params *p) {
   const size_t esz = p->precision == GPE_FP64 ? 16 : 8;
   size_t fields = 2 + n_work_fields(p->scheme);  // psi, psi'
   if (p->precision == GPE_FP32C) fields += 2;     // lo, lo'
+  // the compensated form keeps no rk45 error scale
+  if (p->precision == GPE_FP32C && p->scheme == GPE_RK45) fields -= 1;
   // round each field to 256 bytes
   return fields * ((n * esz + 255) & ~(size_t)255);
 }
@@ -234,6 +236,7 @@ StageArgs inst_args(const Inst &I, int kind, const gpe_step_state &st) {
   A.hi = 0.5 * st.dt_im;
   A.sdtr = st.dt_re;
   A.sdti = st.dt_im;
+  A.inv_absdt = 1.0 / std::hypot(st.dt_re, st.dt_im);
   A.write_scale = 1;
   A.psi = I.buf[0];
   A.psi_out = I.buf[1];
@@ -581,7 +584,7 @@ int gpe_batch_add(gpe_batch *b, const gpe_params *params, const void *psi, int32
       I.acc = p + 2 * fb;
     } else if (params->scheme == GPE_RK45) {
       for (int m = 0; m < 5; ++m) I.k[m] = p + m * fb;
-      I.scale = p + 5 * fb;
+      if (params->precision != GPE_FP32C) I.scale = p + 5 * fb;
     }
     rc = field_upload(params->precision, (size_t)params->nx * params->ny, params->nz, psi, host_type,
                       (size_t)params->nx * params->ny, 0, I.buf[0], I.lob[0]);

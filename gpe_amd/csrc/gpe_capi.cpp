@@ -258,6 +258,7 @@ StageArgs args_for(gpe_ctx *c) {
   A.dti = c->st.dt_im;
   A.hr = 0.5 * c->st.dt_re;
   A.hi = 0.5 * c->st.dt_im;
+  A.inv_absdt = 1.0 / std::hypot(c->st.dt_re, c->st.dt_im);
   A.part_norm = c->part;
   A.part_emax = c->part + c->part_cap;
   A.part_energy = c->part + 2 * c->part_cap;
@@ -556,7 +557,8 @@ int gpe_ctx_create(const gpe_params *params, gpe_ctx **out) {
   for (int m = 0; m < nk; ++m)
     if ((rc = alloc_field(c, &c->k[m]))) goto bad;
   if (params->scheme == GPE_RK4 && (rc = alloc_field(c, &c->acc))) goto bad;
-  if (params->scheme == GPE_RK45 && (rc = alloc_field(c, &c->scale))) goto bad;
+  // the compensated form forms the rk45 error scale in the last stage
+  if (params->scheme == GPE_RK45 && !c->comp && (rc = alloc_field(c, &c->scale))) goto bad;
   {
     // default launch configuration; partial buffers sized for the largest
     // grid any variant and z-chunk can produce

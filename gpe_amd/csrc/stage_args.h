@@ -62,6 +62,7 @@ struct StageArgs {
   double dtr, dti;   // dt
   double hr, hi;     // 0.5*dt
   double sdtr, sdti; // dt at entry of the rk45 step (scale)
+  double inv_absdt;  // 1/|dt| (compensated rk45: |f| = |kk1|/|dt|)
   double b[5];       // Cash-Karp b_{s,1..s-1} of this stage
   double c[7], dc[7];
   // norm weights (variables.f90:49-51)

@@ -245,15 +245,26 @@ struct FifoInfo {
   static constexpr int nf = KIND == K_RK2_2   ? 1   // psi
                             : KIND == K_RK4_2 ? 1   // kk1/6
                             : KIND == K_RK4_4 ? 2   // psi, acc + kk3/3
-                            : KIND == K_CK_6  ? (COMP ? 3 : 2)  // psi_hi, sum c kk, sum dc kk | psi + sum c kk, sum dc kk
+                            : KIND == K_CK_6  ? (COMP ? 4 : 2)  // psi_hi, sum c kk, sum dc kk, scale | psi + sum c kk, sum dc kk
                                               : 0;
 };
+
+// The compensated form forms the rk45 error scale |psi| + dt_step |f(psi)|
+// + 1e-30 in the last stage, from psi and kk1 = dt f(psi) it reads anyway
+// (|f| = |kk1| / |dt|), instead of storing it in stage 1 and reading it
+// back: one field pass less per attempt.  FP64 and plain FP32 store it as
+// the reference computes it (solve.f90:176-178).
+template <typename T>
+__device__ __forceinline__ cx<T> scale_from_kk1(const StageArgs &A, cx<T> ps, cx<T> kk1) {
+  const T af = cabs_(kk1) * (T)A.inv_absdt, au = cabs_(ps);
+  return mk((au + (T)A.sdtr * af) + (T)1e-30, (T)A.sdti * af);
+}
 
 // Pointwise fields read only at the output plane: the rk45 error scale and
 // the compensated low word.
 template <int KIND, bool COMP>
 struct OutInfo {
-  static constexpr int no = (KIND == K_CK_6 ? 1 : 0) + ((COMP && KindInfo<KIND>::final_) ? 1 : 0);
+  static constexpr int no = ((KIND == K_CK_6 && !COMP) ? 1 : 0) + ((COMP && KindInfo<KIND>::final_) ? 1 : 0);
 };
 
 // Stage kinds reading at most this many fields per site prefetch two planes.
@@ -390,6 +401,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
         F[0] = r.ps;
         F[1] = pd;
         F[2] = pe;
+        F[3] = scale_from_kk1(A, r.ps, r.kk[0]);
       } else {
         cx<T> pn = r.ps;
 #pragma unroll
@@ -401,7 +413,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
   };
   auto load_out = [&](size_t s, cx<T> *o) {
     int i = 0;
-    if constexpr (KIND == K_CK_6) {
+    if constexpr (KIND == K_CK_6 && !COMP) {
       // real time: the scale is real (its imaginary part is 0*|f|), stored as one word
       if (A.scale_real)
         o[i++] = mk(static_cast<const T *>(A.scale_in)[s], (T)0);
@@ -570,21 +582,23 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
             }
             if constexpr (KIND == K_CK_1) {
               stc<T>(A.kout, s, kk);
-              if (A.write_scale) {
-                // psi_scal = abs(in) + dt*abs(f) + 1e-30 (solve.f90:176-178)
-                T af = cabs_(f), au = cabs_(c0);
-                cx<T> sc = mk((au + (T)A.sdtr * af) + (T)1e-30, (T)A.sdti * af);
-                if (A.scale_real)
-                  static_cast<T *>(A.scale_out)[s] = sc.re;
-                else
-                  stc<T>(A.scale_out, s, sc);
+              if constexpr (!COMP) {
+                if (A.write_scale) {
+                  // psi_scal = abs(in) + dt*abs(f) + 1e-30 (solve.f90:176-178)
+                  T af = cabs_(f), au = cabs_(c0);
+                  cx<T> sc = mk((au + (T)A.sdtr * af) + (T)1e-30, (T)A.sdti * af);
+                  if (A.scale_real)
+                    static_cast<T *>(A.scale_out)[s] = sc.re;
+                  else
+                    stc<T>(A.scale_out, s, sc);
+                }
               }
             } else if constexpr (KIND < K_CK_6) {
               stc<T>(A.kout, s, kk);
             } else {
               // K_CK_6: new state and the error norm (solve.f90:365-375, 184)
-              const cx<T> sc = oc[p][0];
-              const cx<T> err = F[0][p][NF - 1] + rmul((T)A.dc[6], kk);
+              const cx<T> sc = COMP ? F[0][p][NF - 1] : oc[p][0];
+              const cx<T> err = F[0][p][COMP ? 2 : 1] + rmul((T)A.dc[6], kk);
               cx<double> np;
               if constexpr (IS_DBL) {
                 np = F[0][p][0] + rmul((T)A.c[6], kk);
@@ -594,7 +608,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
               } else {
                 if constexpr (COMP) {
                   const cx<double> dl = todbl(F[0][p][1]) + rmul(A.c[6], todbl(kk));
-                  np = (todbl(F[0][p][0]) + todbl(oc[p][1])) + dl;
+                  np = (todbl(F[0][p][0]) + todbl(oc[p][0])) + dl;
                   const cx<T> hi = mk((T)np.re, (T)np.im);
                   stc<T>(A.psi_out, s, hi);
                   stc<T>(A.lo_out, s, mk((T)(np.re - (double)hi.re), (T)(np.im - (double)hi.im)));
