@@ -167,7 +167,7 @@ def batch_leg(seed, steps=None, log_=log):
         done = dict(b.poll())
         assert len(done) == len(ids) and all(v == 1 for v in done.values()), done
         got = [b.download(i) for i in ids]
-        tiles = {k: b.tune_get("fp32c", 2, "rk4", k) for k in ("rk4_1", "rk4_2", "rk4_3", "rk4_4")}
+        tiles = {k: b.tune_get("fp32c", 2, "rk4", k) for k in ("rk4_1", "rk4_2", "rk4_3", "rk4_4", "rk4_12", "rk4_34")}
     log_("batch fp32c: %d solves in %.3f s" % (len(ids), el_total))
     # sequential plain FP64, one simulation after another
     seq = 0.0

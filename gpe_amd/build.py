@@ -39,10 +39,9 @@ for _v in range(N_VARIANTS):
     # into different kernels) compute bitwise the same field
     UNITS.append(("gpe_stage.hip", "stage_f32_v%d" % _v, ["-ffp-contract=off", "-DGPE_VID=%d" % _v, "-DGPE_NT_STORE"]))
 
-# fused stage pairs: three tile variants in FP32, one in FP64
+# fused stage pairs: FP32 forms only (three tile variants)
 for _v in range(3):
     UNITS.append(("gpe_pair.hip", "pair_f32_v%d" % _v, ["-ffp-contract=off", "-DGPE_PVID=%d" % _v, "-DGPE_NT_STORE"]))
-UNITS.append(("gpe_pair.hip", "pair_f64_v0", ["-ffp-contract=off", "-DGPE_F64", "-DGPE_PVID=0", "-DGPE_NT_STORE"]))
 
 
 def _compile(unit):

@@ -85,14 +85,47 @@ struct KindTable {
   std::vector<WorkItem> h;
 };
 
+// Fused stage pairs (fused.hpp) and their first stage kinds.
+constexpr int kPairFirst[P_COUNT] = {K_RK4_1, K_RK4_3, K_CK_1, K_CK_3, K_CK_5};
+
+int pair_at(int kind) {
+  for (int q = 0; q < P_COUNT; ++q)
+    if (kPairFirst[q] == kind) return q;
+  return -1;
+}
+
 struct Group {
   int prec = GPE_FP64, order = 2, scheme = GPE_RK4;
   int R = 1, k0 = 0, nk = 0;
   int vid[K_COUNT], zc[K_COUNT];
   KindTable tab[K_COUNT];
+  // pairs run fused where the autotuner found that faster
+  bool pfuse[P_COUNT];
+  int pvid[P_COUNT], pzc[P_COUNT];
+  KindTable ptab[P_COUNT];
   std::vector<int> slots;  // instance slots in this group, ascending
   int part_off = 0;        // first partial slot of this group's final stage
 };
+
+// One launch of an iteration: a stage kind, or a fused pair (first kind k).
+struct Unit {
+  int kind, pair;  // pair = -1 for a single stage
+  bool final_;
+};
+
+std::vector<Unit> units_of(const Group &G) {
+  std::vector<Unit> u;
+  for (int k = G.k0; k < G.k0 + G.nk; ++k) {
+    const int q = pair_at(k);
+    if (q >= 0 && G.pfuse[q] && k + 1 < G.k0 + G.nk) {
+      u.push_back({k, q, k + 1 == G.k0 + G.nk - 1});
+      ++k;
+    } else {
+      u.push_back({k, -1, k == G.k0 + G.nk - 1});
+    }
+  }
+  return u;
+}
 
 }  // namespace
 
@@ -142,6 +175,11 @@ int group_of(gpe_batch *b, int prec, int order, int scheme) {
   for (int k = 0; k < K_COUNT; ++k) {
     G.vid[k] = kDefaultVid;
     G.zc[k] = kDefaultZc;
+  }
+  for (int q = 0; q < P_COUNT; ++q) {
+    G.pfuse[q] = false;
+    G.pvid[q] = 0;
+    G.pzc[q] = 0;
   }
   b->groups.push_back(G);
   return (int)b->groups.size() - 1;
@@ -284,18 +322,18 @@ StageArgs inst_args(const Inst &I, int kind, const gpe_step_state &st) {
 // Work table of one kind of a group: every tile of every instance, each
 // instance's partial slots contiguous in (z, y, x) tile order; the table
 // itself is ordered largest tile first so the persistent grid ends evenly.
-void make_table(gpe_batch *b, const Group &G, int kind, int vid, int zc, std::vector<WorkItem> &out,
+// Work table over output tiles of TX x TY sites (a stage kind: 64 x TY; a
+// fused pair: the (64-2R) x (TY-2R) interiors).
+void make_table(gpe_batch *b, const Group &G, int TX, int TY, int zc, std::vector<WorkItem> &out,
                 std::vector<int> *range, int part_off) {
   out.clear();
-  const Tile t = kTiles[vid];
-  const int TY = t.nw * t.p;
   int slotc = part_off;
   std::vector<long long> cost;
   for (int s : G.slots) {
     const Inst &I = b->slot[s];
     const int nx = I.P.nx, ny = I.P.ny, nz = I.P.nz;
     const int zcl = zc > 0 ? std::min(zc, nz) : nz;
-    const int ntx = (nx + 63) / 64, nty = (ny + TY - 1) / TY;
+    const int ntx = (nx + TX - 1) / TX, nty = (ny + TY - 1) / TY;
     const int beg = slotc;
     for (int z0 = 0; z0 < nz; z0 += zcl) {
       const int z1 = std::min(z0 + zcl, nz);
@@ -309,7 +347,7 @@ void make_table(gpe_batch *b, const Group &G, int kind, int vid, int zc, std::ve
           w.z1 = z1;
           w.slot = slotc++;
           out.push_back(w);
-          const long long cx = std::min(64, nx - 64 * tx), cy = std::min(TY, ny - TY * ty);
+          const long long cx = std::min(TX, nx - TX * tx), cy = std::min(TY, ny - TY * ty);
           cost.push_back((long long)(cx * cy) * (z1 - z0 + 2 * G.R));
         }
     }
@@ -318,13 +356,22 @@ void make_table(gpe_batch *b, const Group &G, int kind, int vid, int zc, std::ve
       (*range)[2 * s + 1] = slotc;
     }
   }
-  (void)kind;
   std::vector<int> ord(out.size());
   for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return cost[a] > cost[c]; });
   std::vector<WorkItem> sorted(out.size());
   for (size_t i = 0; i < ord.size(); ++i) sorted[i] = out[ord[i]];
   out.swap(sorted);
+}
+
+void kind_tiles(int vid, int *TX, int *TY) {
+  *TX = 64;
+  *TY = kTiles[vid].nw * kTiles[vid].p;
+}
+
+void pair_tiles(const Group &G, int vid, int *TX, int *TY) {
+  *TX = 64 - 2 * G.R;
+  *TY = kPairTiles[vid].nw * kPairTiles[vid].p - 2 * G.R;
 }
 
 int upload_table(KindTable &T) {
@@ -368,11 +415,17 @@ int rebuild(gpe_batch *b) {
     for (int s = 0; s < b->cap; ++s)
       if (b->slot[s].id >= 0 && b->slot[s].group == (int)(&G - &b->groups[0])) G.slots.push_back(s);
     G.part_off = part;
-    for (int k = G.k0; k < G.k0 + G.nk; ++k) {
-      const bool fin = k == G.k0 + G.nk - 1;
-      make_table(b, G, k, G.vid[k], G.zc[k], G.tab[k].h, fin ? &range : nullptr, part);
-      if ((rc = upload_table(G.tab[k]))) return rc;
-      if (fin) part += (int)G.tab[k].h.size();
+    for (const Unit &u : units_of(G)) {
+      int TX, TY;
+      KindTable &T = u.pair >= 0 ? G.ptab[u.pair] : G.tab[u.kind];
+      if (u.pair >= 0)
+        pair_tiles(G, G.pvid[u.pair], &TX, &TY);
+      else
+        kind_tiles(G.vid[u.kind], &TX, &TY);
+      make_table(b, G, TX, TY, u.pair >= 0 ? G.pzc[u.pair] : G.zc[u.kind], T.h, u.final_ ? &range : nullptr,
+                 part);
+      if ((rc = upload_table(T))) return rc;
+      if (u.final_) part += (int)T.h.size();
     }
   }
   if ((rc = ensure_partials(b, std::max(part, 1)))) return rc;
@@ -396,19 +449,33 @@ int enqueue_iteration(gpe_batch *b, bool fork) {
       s = b->aux[j % kAuxStreams];
       GPE_HIPCHK(hipStreamWaitEvent(s, b->ev_fork, 0));
     }
-    for (int k = G.k0; k < G.k0 + G.nk; ++k) {
-      const KindTable &T = G.tab[k];
+    for (const Unit &u : units_of(G)) {
+      const int k = u.kind;
+      const KindTable &T = u.pair >= 0 ? G.ptab[u.pair] : G.tab[k];
       if (!T.n) continue;
-      const bool fin = k == G.k0 + G.nk - 1;
-      BatchLaunch L;
-      L.insts = b->d_args + (size_t)k * b->cap;
-      L.work = T.d;
-      L.nwork = T.n;
-      L.active = b->d_active;
-      L.pn = fin ? b->d_pn : nullptr;
-      L.pe = fin ? b->d_pe : nullptr;
-      L.grid = T.grid;
-      hipError_t e = launch_stage_batched(G.prec == GPE_FP64 ? 16 : 8, G.vid[k], k, G.R, G.prec == GPE_FP32C, L, s);
+      hipError_t e;
+      if (u.pair >= 0) {
+        PairBatchLaunch L;
+        L.instA = b->d_args + (size_t)k * b->cap;
+        L.instB = b->d_args + (size_t)(k + 1) * b->cap;
+        L.work = T.d;
+        L.nwork = T.n;
+        L.active = b->d_active;
+        L.pn = u.final_ ? b->d_pn : nullptr;
+        L.pe = u.final_ ? b->d_pe : nullptr;
+        L.grid = T.grid;
+        e = launch_pair_batched(G.prec == GPE_FP64 ? 16 : 8, G.pvid[u.pair], u.pair, G.R, G.prec == GPE_FP32C, L, s);
+      } else {
+        BatchLaunch L;
+        L.insts = b->d_args + (size_t)k * b->cap;
+        L.work = T.d;
+        L.nwork = T.n;
+        L.active = b->d_active;
+        L.pn = u.final_ ? b->d_pn : nullptr;
+        L.pe = u.final_ ? b->d_pe : nullptr;
+        L.grid = T.grid;
+        e = launch_stage_batched(G.prec == GPE_FP64 ? 16 : 8, G.vid[k], k, G.R, G.prec == GPE_FP32C, L, s);
+      }
       if (e != hipSuccess) return fail(GPE_EHIP, "batched stage launch: %s", hipGetErrorString(e));
     }
     if (fork) GPE_HIPCHK(hipEventRecord(b->ev_done[j], s));
@@ -764,12 +831,25 @@ int gpe_batch_use_graph(gpe_batch *b, int32_t enable) {
 int gpe_batch_tune_set(gpe_batch *b, int32_t precision, int32_t order, int32_t scheme, int32_t kind,
                        int32_t variant, int32_t zchunk) {
   if (!b) return fail(GPE_EINVAL, "batch is NULL");
-  if (variant < 0 || variant >= N_TILE_VARIANTS) return fail(GPE_EINVAL, "variant out of range");
+  if (kind < 16 && (variant < 0 || variant >= N_TILE_VARIANTS)) return fail(GPE_EINVAL, "variant out of range");
   if (zchunk < 0 || (zchunk > 0 && zchunk < 8)) return fail(GPE_EINVAL, "zchunk must be 0 or >= 8");
   if ((precision != GPE_FP64 && precision != GPE_FP32C) || (order != 2 && order != 4))
     return fail(GPE_EINVAL, "bad precision or order");
   GPE_HIPCHK(hipStreamSynchronize(b->stream));
   Group &G = b->groups[group_of(b, precision, order, scheme)];
+  if (kind >= 16 && kind < 16 + P_COUNT) {
+    // a fused pair: variant -1 runs it as its two stages
+    const int q = kind - 16;
+    if (kPairFirst[q] < G.k0 || kPairFirst[q] + 1 >= G.k0 + G.nk)
+      return fail(GPE_EINVAL, "pair %d is not in scheme %d", kind, scheme);
+    const int nv = precision == GPE_FP64 ? N_PAIR_VARIANTS_F64 : N_PAIR_VARIANTS;
+    if (variant >= nv) return fail(GPE_EINVAL, "pair variant out of range (FP64 has no fused pairs)");
+    G.pfuse[q] = variant >= 0;
+    if (variant >= 0) G.pvid[q] = variant;
+    G.pzc[q] = zchunk;
+    b->dirty = true;
+    return GPE_OK;
+  }
   if (kind < G.k0 || kind >= G.k0 + G.nk) return fail(GPE_EINVAL, "kind %d is not a stage of scheme %d", kind, scheme);
   G.vid[kind] = variant;
   G.zc[kind] = zchunk;
@@ -782,13 +862,19 @@ int gpe_batch_tune_get(gpe_batch *b, int32_t precision, int32_t order, int32_t s
   if (!b) return fail(GPE_EINVAL, "batch is NULL");
   for (const Group &G : b->groups)
     if (G.prec == precision && G.order == order && G.scheme == scheme) {
+      if (kind >= 16 && kind < 16 + P_COUNT) {
+        const int q = kind - 16;
+        if (variant) *variant = G.pfuse[q] ? G.pvid[q] : -1;
+        if (zchunk) *zchunk = G.pzc[q];
+        return GPE_OK;
+      }
       if (kind < G.k0 || kind >= G.k0 + G.nk) return fail(GPE_EINVAL, "kind %d is not a stage of scheme %d", kind, scheme);
       if (variant) *variant = G.vid[kind];
       if (zchunk) *zchunk = G.zc[kind];
       return GPE_OK;
     }
-  if (variant) *variant = kDefaultVid;
-  if (zchunk) *zchunk = kDefaultZc;
+  if (variant) *variant = kind >= 16 ? -1 : kDefaultVid;
+  if (zchunk) *zchunk = kind >= 16 ? 0 : kDefaultZc;
   return GPE_OK;
 }
 
@@ -808,6 +894,8 @@ int gpe_batch_autotune(gpe_batch *b, int32_t reps) {
   static const int zcs[] = {8, 16, 32, 64, 0};
   for (Group &G : b->groups) {
     if (G.slots.empty()) continue;
+    float kbest[K_COUNT];
+    for (float &v : kbest) v = 1e30f;
     for (int k = G.k0; k < G.k0 + G.nk; ++k) {
       const bool fin = k == G.k0 + G.nk - 1;
       float best = 1e30f;
@@ -815,7 +903,9 @@ int gpe_batch_autotune(gpe_batch *b, int32_t reps) {
       KindTable T;
       for (int v = 0; v < N_TILE_VARIANTS; ++v)
         for (int z : zcs) {
-          make_table(b, G, k, v, z, T.h, nullptr, 0);
+          int TX, TY;
+          kind_tiles(v, &TX, &TY);
+          make_table(b, G, TX, TY, z, T.h, nullptr, 0);
           if ((rc = upload_table(T)) || (rc = ensure_partials(b, T.n))) break;
           BatchLaunch L;
           L.insts = b->d_args + (size_t)k * b->cap;
@@ -848,6 +938,57 @@ int gpe_batch_autotune(gpe_batch *b, int32_t reps) {
       if (rc) break;
       G.vid[k] = bv;
       G.zc[k] = bz;
+      kbest[k] = best;
+    }
+    if (rc) break;
+    // fused pairs (FP32 forms): kept where >= 3% faster than their two stages
+    for (int k = G.k0; k + 1 < G.k0 + G.nk && rc == GPE_OK && G.prec != GPE_FP64; ++k) {
+      const int q = pair_at(k);
+      if (q < 0) continue;
+      const bool fin = k + 1 == G.k0 + G.nk - 1;
+      const int nv = G.prec == GPE_FP64 ? N_PAIR_VARIANTS_F64 : N_PAIR_VARIANTS;
+      float best = 1e30f;
+      int bv = G.pvid[q], bz = G.pzc[q];
+      KindTable T;
+      for (int v = 0; v < nv && rc == GPE_OK; ++v)
+        for (int z : {0, 32, 64}) {
+          int TX, TY;
+          pair_tiles(G, v, &TX, &TY);
+          make_table(b, G, TX, TY, z, T.h, nullptr, 0);
+          if ((rc = upload_table(T)) || (rc = ensure_partials(b, T.n))) break;
+          PairBatchLaunch L;
+          L.instA = b->d_args + (size_t)k * b->cap;
+          L.instB = b->d_args + (size_t)(k + 1) * b->cap;
+          L.work = T.d;
+          L.nwork = T.n;
+          L.active = b->d_active;
+          L.pn = fin ? b->d_pn : nullptr;
+          L.pe = fin ? b->d_pe : nullptr;
+          L.grid = T.grid;
+          const int pb = G.prec == GPE_FP64 ? 16 : 8;
+          const bool comp = G.prec == GPE_FP32C;
+          hipError_t e = launch_pair_batched(pb, v, q, G.R, comp, L, b->stream);  // warm
+          e = e != hipSuccess ? e : hipEventRecord(e0, b->stream);
+          for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_pair_batched(pb, v, q, G.R, comp, L, b->stream);
+          e = e != hipSuccess ? e : hipEventRecord(e1, b->stream);
+          e = e != hipSuccess ? e : hipEventSynchronize(e1);
+          if (e != hipSuccess) {
+            rc = fail(GPE_EHIP, "autotune launch: %s", hipGetErrorString(e));
+            break;
+          }
+          float ms = 0.f;
+          hipEventElapsedTime(&ms, e0, e1);
+          if (ms < best) {
+            best = ms;
+            bv = v;
+            bz = z;
+          }
+        }
+      if (T.d) hipFree(T.d);
+      G.pvid[q] = bv;
+      G.pzc[q] = bz;
+      G.pfuse[q] = best < 0.97f * (kbest[k] + kbest[k + 1]);
+      ++k;  // pairs do not overlap
     }
     if (rc) break;
   }

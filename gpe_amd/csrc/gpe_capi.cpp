@@ -862,6 +862,7 @@ int gpe_solver_step(gpe_ctx *c, const void *var_in, int32_t halo, void *var_out,
 
 int gpe_set_fusion(gpe_ctx *c, int32_t enable) {
   if (!c) return fail(GPE_EINVAL, "NULL context");
+  if (enable && c->prec == GPE_FP64) return fail(GPE_EINVAL, "FP64 runs its stages one by one (no fused pairs)");
   c->fuse = enable != 0;
   return GPE_OK;
 }

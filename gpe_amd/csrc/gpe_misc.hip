@@ -53,9 +53,7 @@ hipError_t launch_stage_batched(int prec_bytes, int vid, int kind, int radius, b
 
 hipError_t launch_pair(int prec_bytes, int vid, int pair, int radius, bool comp, const StageArgs &A,
                        const StageArgs &B, hipStream_t s) {
-  if (prec_bytes == 16) {
-    if (vid == 0) return launch_pair_f64_v0(pair, radius, comp, A, B, s);
-  } else {
+  if (prec_bytes != 16) {
     switch (vid) {
       case 0: return launch_pair_f32_v0(pair, radius, comp, A, B, s);
       case 1: return launch_pair_f32_v1(pair, radius, comp, A, B, s);
@@ -67,9 +65,7 @@ hipError_t launch_pair(int prec_bytes, int vid, int pair, int radius, bool comp,
 
 hipError_t launch_pair_batched(int prec_bytes, int vid, int pair, int radius, bool comp, const PairBatchLaunch &L,
                                hipStream_t s) {
-  if (prec_bytes == 16) {
-    if (vid == 0) return launch_pair_batched_f64_v0(pair, radius, comp, L, s);
-  } else {
+  if (prec_bytes != 16) {
     switch (vid) {
       case 0: return launch_pair_batched_f32_v0(pair, radius, comp, L, s);
       case 1: return launch_pair_batched_f32_v1(pair, radius, comp, L, s);

@@ -1,18 +1,13 @@
-// gpe_pair.hip -- fused stage-pair kernels of one precision and one tile
-// variant (see fused.hpp).  Compiled once per (GPE_F64 or not, GPE_PVID).
+// gpe_pair.hip -- fused stage-pair kernels (FP32 forms) of one tile variant
+// (see fused.hpp).  Compiled once per GPE_PVID.
 #include "fused.hpp"
 #include "launch.h"
 
 #ifndef GPE_PVID
 #error "GPE_PVID (pair tile variant) must be defined"
 #endif
-#ifdef GPE_F64
-#define GPE_REAL double
-#define GPE_PNAME f64
-#else
 #define GPE_REAL float
 #define GPE_PNAME f32
-#endif
 #define GPE_CAT3(a, b, c) a##b##c
 #define GPE_PFN(p, v) GPE_CAT3(launch_pair_, p, _v##v)
 #define GPE_PFN2(p, v) GPE_PFN(p, v)
@@ -67,26 +62,14 @@ hipError_t launch_rb(int pair, const PairBatchLaunch &L, hipStream_t s) {
 
 hipError_t GPE_PFN2(GPE_PNAME, GPE_PVID)(int pair, int radius, bool comp, const StageArgs &A, const StageArgs &B,
                                          hipStream_t s) {
-#ifdef GPE_F64
-  (void)comp;
-  if (radius == 1) return launch_r<1, false>(pair, A, B, s);
-  if (radius == 2) return launch_r<2, false>(pair, A, B, s);
-#else
   if (radius == 1) return comp ? launch_r<1, true>(pair, A, B, s) : launch_r<1, false>(pair, A, B, s);
   if (radius == 2) return comp ? launch_r<2, true>(pair, A, B, s) : launch_r<2, false>(pair, A, B, s);
-#endif
   return hipErrorInvalidValue;
 }
 
 hipError_t GPE_PBFN2(GPE_PNAME, GPE_PVID)(int pair, int radius, bool comp, const PairBatchLaunch &L, hipStream_t s) {
-#ifdef GPE_F64
-  (void)comp;
-  if (radius == 1) return launch_rb<1, false>(pair, L, s);
-  if (radius == 2) return launch_rb<2, false>(pair, L, s);
-#else
   if (radius == 1) return comp ? launch_rb<1, true>(pair, L, s) : launch_rb<1, false>(pair, L, s);
   if (radius == 2) return comp ? launch_rb<2, true>(pair, L, s) : launch_rb<2, false>(pair, L, s);
-#endif
   return hipErrorInvalidValue;
 }
 

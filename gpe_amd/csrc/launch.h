@@ -30,9 +30,11 @@ struct PairTile {
   int nw, p, d;
 };
 constexpr PairTile kPairTiles[N_PAIR_VARIANTS] = {{8, 2, 2}, {16, 1, 2}, {16, 1, 4}};
-// Variants compiled for FP64 (the fused FP64 kernels exist to check that
-// fusion changes no bit; the FP64 throughput path runs the stages one by one)
-constexpr int N_PAIR_VARIANTS_F64 = 1;
+// FP64 (the parity path) always runs the stages one by one: its fused
+// kernels would spill ~1 KB per lane to scratch, gain nothing, and a
+// batched 4th-order FP64 build gave results different from the
+// stage-by-stage sweeps for reasons not found in the source.
+constexpr int N_PAIR_VARIANTS_F64 = 0;
 // A: stage A's descriptor, B: stage B's (B carries the grid: ntx/nty over
 // the (64-2R) x (TY-2R) interiors, zc, nblocks).
 hipError_t launch_pair(int prec_bytes, int vid, int pair, int radius, bool comp, const StageArgs &A,
@@ -118,7 +120,6 @@ GPE_DECL_BVARIANT(f32, 4)
                                     hipStream_t s);                                                         \
   hipError_t launch_pair_batched_##P##_v##V(int pair, int radius, bool comp, const PairBatchLaunch &L,       \
                                             hipStream_t s);
-GPE_DECL_PVARIANT(f64, 0)
 GPE_DECL_PVARIANT(f32, 0)
 GPE_DECL_PVARIANT(f32, 1)
 GPE_DECL_PVARIANT(f32, 2)

@@ -211,14 +211,16 @@ int gpe_batch_stats(gpe_batch *b, int32_t *instances, int32_t *running, int64_t 
                     int64_t *budget);
 /* 1: replay the captured iteration graph (default); 0: launch kernels. */
 int gpe_batch_use_graph(gpe_batch *b, int32_t enable);
-/* Tile variant and z-chunk (0 = whole column) of one stage kind for the
- * instances of (precision, order, scheme). */
+/* Tile variant and z-chunk (0 = whole column) of one stage kind (0 .. 14)
+ * or fused stage pair (16 .. 20, variant -1 = run as its two stages) for
+ * the instances of (precision, order, scheme). */
 int gpe_batch_tune_set(gpe_batch *b, int32_t precision, int32_t order, int32_t scheme, int32_t kind,
                        int32_t variant, int32_t zchunk);
 int gpe_batch_tune_get(gpe_batch *b, int32_t precision, int32_t order, int32_t scheme, int32_t kind,
                        int32_t *variant, int32_t *zchunk);
-/* Time every variant and z-chunk of every stage kind over the current
- * instances and keep the fastest.  Commits no step. */
+/* Time every variant and z-chunk of every stage kind and stage pair over
+ * the current instances and keep the fastest (a pair runs fused only where
+ * that beats its two stages).  Commits no step. */
 int gpe_batch_autotune(gpe_batch *b, int32_t reps);
 
 #ifdef __cplusplus

@@ -243,3 +243,32 @@ def test_budget_admission():
         b.run_to_completion()
         b.remove(a)
         assert b.add(c, ring(c), 3) is not None  # freed memory is reused
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,scheme,variant", [("fp32c", "rk4", 0), ("fp32c", "rk4", 1), ("fp32c", "rk4", 2),
+                                                      ("fp32c", "rk45", 0), ("fp32c", "rk45", 1), ("fp32c", "rk45", 2)])
+def test_batch_fused_pairs_equal_unfused(precision, scheme, variant):
+    """Batched fused stage pairs give bitwise the batch's stage-by-stage
+    results (mixed orders: two groups)."""
+    from gpe_amd.batch import Batch
+    tau = 0.05 if scheme == "rk45" else 0.005
+    cfgs = [cfg_of(m["n"], scheme=scheme, order=m["order"], bcs=m["bcs"], precision=precision, om=m["om"],
+                   tau=tau) for m in MIX]
+    psis = [ring(c) for c in cfgs]
+    pairs = ["rk4_12", "rk4_34"] if scheme == "rk4" else ["ck_12", "ck_34", "ck_56"]
+    say("batch fused pairs", precision, scheme, variant)
+    res = []
+    for fused in (False, True):
+        with Batch() as b:
+            if fused:
+                for order in (2, 4):
+                    for pr in pairs:
+                        b.tune_set(precision, order, scheme, pr, variant, 16)
+            ids = [b.add(c, p, 6) for c, p in zip(cfgs, psis)]
+            b.run_to_completion(chunk=4)
+            res.append([b.download(i) for i in ids])
+            if fused:
+                assert b.tune_get(precision, 2, scheme, pairs[0]) == (variant, 16)
+    for a, c in zip(res[0], res[1]):
+        assert np.array_equal(a, c)

@@ -3,9 +3,9 @@
 The fused kernels compute every value with the same expressions in the same
 order as the one-stage kernels, so the fields, times and step sizes must be
 bitwise identical -- for both boundary conditions (the reflective walls
-exercise the mirror-partner handling in x, y and z), both orders, every
-precision and every pair tile variant, with short z-chunks so chunk seams
-are crossed.  FP64 additionally stays within 1e-12 of the oracle."""
+exercise the mirror-partner handling in x, y and z), both orders, both FP32
+forms and every pair tile variant, with short z-chunks so chunk seams are
+crossed.  FP64, the parity path, has no fused pairs."""
 import os
 import sys
 
@@ -53,7 +53,7 @@ def _run(cfg, psi, n, fused, tune=None):
 SHAPES = [(70, 21, 19), (16, 16, 16), (40, 9, 25)]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32c"])
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
 @pytest.mark.parametrize("scheme", ["rk4", "rk45"])
 @pytest.mark.parametrize("order", [2, 4])
 @pytest.mark.parametrize("bcs", [1, 2])
@@ -82,15 +82,20 @@ def test_pair_variants_and_chunks(variant, scheme):
     assert np.array_equal(a, b)
 
 
-def test_fused_fp64_vs_oracle(oracle):
+def test_fused_fp32c_vs_oracle_and_fp64_refuses(oracle):
+    from gpe_amd import _lib
     from gpe_amd.sim import Simulation
-    cfg = cfg_of((70, 21, 19), scheme="rk45", order=4, bcs=2, tau=0.05)
+    cfg = cfg_of((70, 21, 19), scheme="rk4", order=4, bcs=2)
     psi = ic_of(cfg)
-    say("fused fp64 rk45 vs oracle")
-    st = oracle.make_state(cfg.dt)
-    want, _ = oracle.run(cfg, st, psi, 5, NT)
-    with Simulation(cfg, psi) as sim:
+    say("fused fp32c rk4 vs oracle; fp64 has no fusion")
+    want, _ = oracle.run(cfg, oracle.make_state(cfg.dt), psi, 20, NT)
+    with Simulation(cfg.with_(precision="fp32c"), psi) as sim:
         sim.set_fusion(True)
-        sim.advance(5)
-        got = sim.download()
-    assert float(np.linalg.norm(got - want) / np.linalg.norm(want)) < 1e-12
+        for p in ("rk4_12", "rk4_34"):
+            sim.tune_set(p, 0, 0)
+        sim.advance(20)
+        got = sim.download(np.complex128)
+    assert float(np.linalg.norm(got - want) / np.linalg.norm(want)) < 1e-6
+    with Simulation(cfg, psi) as sim:
+        with pytest.raises(_lib.GpeError, match="one by one"):
+            sim.set_fusion(True)
