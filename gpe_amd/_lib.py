@@ -5,6 +5,7 @@ raises.  Build it with `python -m gpe_amd.build`.
 """
 import ctypes
 import os
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("GPE_LIBRARY") or os.path.join(HERE, "libgpehip.so")
@@ -27,12 +28,17 @@ KIND_NAMES = ["rhs", "euler", "rk2_1", "rk2_2", "rk4_1", "rk4_2", "rk4_3", "rk4_
 STAGE_KINDS = KIND_NAMES[:15]
 PAIR_KINDS = KIND_NAMES[16:]
 
+# struct gpe_zero_point (include/gpe_hip.h)
+ZERO_DTYPE = np.dtype([("section", np.int32), ("reserved", np.int32), ("cell", np.int64), ("x", np.float64),
+                       ("y", np.float64), ("z", np.float64)])
+ZERO_FILES = {"zeros": 0, "re_zeros": 1, "im_zeros": 2}
+
 # Exported entry points (must match include/gpe_hip.h).
 EXPORTS = [
     "gpe_last_error", "gpe_ctx_create", "gpe_ctx_destroy", "gpe_solver_step",
     "gpe_sim_upload", "gpe_sim_download", "gpe_sim_set_state", "gpe_sim_get_state",
     "gpe_sim_advance", "gpe_sim_norm_history", "gpe_sim_norm", "gpe_sim_energy",
-    "gpe_sim_mass", "gpe_sim_linelength", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
+    "gpe_sim_mass", "gpe_sim_linelength", "gpe_sim_zeros", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
     "gpe_tune_set", "gpe_set_fusion", "gpe_tune_get", "gpe_autotune", "gpe_probe_bandwidth",
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_footprint", "gpe_batch_add",
     "gpe_batch_remove", "gpe_batch_run", "gpe_batch_sync", "gpe_batch_poll", "gpe_batch_query",
@@ -100,6 +106,7 @@ def lib():
         "gpe_sim_energy": ([vp, dp], ctypes.c_int),
         "gpe_sim_mass": ([vp, dp], ctypes.c_int),
         "gpe_sim_linelength": ([vp, dp], ctypes.c_int),
+        "gpe_sim_zeros": ([vp, i32, vp, i64, ctypes.POINTER(i64)], ctypes.c_int),
         "gpe_sim_renormalise": ([vp, ctypes.c_double, ctypes.c_double], ctypes.c_int),
         "gpe_sim_rhs": ([vp, vp, i32], ctypes.c_int),
         "gpe_prof_enable": ([vp, i32], ctypes.c_int),

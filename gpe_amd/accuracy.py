@@ -8,8 +8,10 @@ in FP32C from the same initial state, and reports for each: simulated time,
 the reference's energy functional (energy, variables.f90:580-604), the
 eqn-4 Hamiltonian H = K + (g/2)int|psi|^4 - mu N that the dynamics conserve,
 the norm, and the vortex-line length (linelength, variables.f90:896-1162);
-then the relative L2 distance of the two fields and the differences of
-those quantities.
+then the relative L2 distance of the two fields, the differences of those
+quantities, and the vortex-core position difference: the distance from
+each core point of one run (the reference's zeros file, io.f90:886-1236)
+to the nearest core point of the other, in units of dx.
 
 With periodic walls int Re(-lap(psi) psi*) = 2K, so the reference functional
 E_ref = 2K + P (P = (1/2) int |psi|^4) gives H = (E_ref + P)/2 - mu N; P and N
@@ -27,7 +29,8 @@ field rounded to complex64 (a relative change of ~3e-8), on the same time
 sequence.
 
 A second report covers the 64^3 vortex ring (dx = 0.4, reflective, 2nd
-order, RK4 dt = 0.01, 1000 steps): FP32C against FP64 after 1000 RK4 steps.
+order, RK4 dt = 0.01, 1000 steps): FP32C against FP64 after 1000 RK4 steps,
+with the ring's radius and height from its core points.
 """
 import argparse
 import json
@@ -58,6 +61,27 @@ def _observables(sim, cfg):
                  "linelength": sim.linelength()}
 
 
+def core_points(pts):
+    """(n, 3) coordinates of the finite points of a zeros pass (the
+    reference's face formula gives NaN where one component's two face
+    crossings share a coordinate)."""
+    xyz = np.stack([pts["x"], pts["y"], pts["z"]], axis=1)
+    return xyz[np.all(np.isfinite(xyz), axis=1)]
+
+
+def core_offsets(pa, pb, dx):
+    """Vortex-core position difference of two fields: the distance from
+    every core point (zeros file, get_zeros + get_extra_zeros) of one field
+    to the nearest core point of the other, both ways, in units of dx."""
+    from scipy.spatial import cKDTree
+    a, b = core_points(pa), core_points(pb)
+    if len(a) == 0 or len(b) == 0:
+        return {"points": [len(a), len(b)]}
+    d = np.concatenate([cKDTree(b).query(a)[0], cKDTree(a).query(b)[0]]) / dx
+    return {"points": [len(a), len(b)], "mean_dx": float(d.mean()), "p99_dx": float(np.quantile(d, 0.99)),
+            "max_dx": float(d.max()), "frac_over_tenth_dx": float((d > 0.1).mean())}
+
+
 def run(n=512, steps=1000, seed=20261202, log=None):
     log = log or (lambda *a: None)
     psi0 = ic.random_phase(workload("fp64", n), seed)
@@ -85,6 +109,7 @@ def run(n=512, steps=1000, seed=20261202, log=None):
                 log("step %d, t = %.6f, %.1f s" % (i + 1, after["t"], time.perf_counter() - t0))
         p64, e64 = _observables(s64, c64)
         p32, e32 = _observables(s32, c32)
+        z64, z32 = s64.zeros(), s32.zeros()
         pct = sct.download()
     out["psi_rel_l2_control_fp64_from_rounded_ic"] = float(np.linalg.norm((pct - p64).ravel()) /
                                                            np.linalg.norm(p64.ravel()))
@@ -98,6 +123,7 @@ def run(n=512, steps=1000, seed=20261202, log=None):
     out["fp64_rejected_attempts"] = rejected
     out["psi_rel_l2"] = float(np.linalg.norm((p32 - p64).ravel()) / np.linalg.norm(p64.ravel()))
     out["linelength_rel_diff"] = (e32["linelength"] - e64["linelength"]) / e64["linelength"]
+    out["core_position"] = core_offsets(z32, z64, c64.dx)
     out["hamiltonian_rel_diff"] = (e32["hamiltonian"] - e64["hamiltonian"]) / abs(e64["hamiltonian"])
     out["energy_ref_rel_diff"] = (e32["energy_ref"] - e64["energy_ref"]) / abs(e64["energy_ref"])
     return out
@@ -114,12 +140,17 @@ def ring_rk4(steps=1000):
         with Simulation(c, psi0) as sim:
             e0 = sim.energy()
             sim.advance(steps)
-            res[prec] = (sim.download(np.complex128), sim.energy(), e0, sim.linelength())
+            res[prec] = (sim.download(np.complex128), sim.energy(), e0, sim.linelength(), sim.zeros())
     a, b = res["fp32c"], res["fp64"]
+    ring = {}
+    for prec, r in res.items():
+        q = core_points(r[4])
+        ring[prec] = {"radius": float(np.hypot(q[:, 0], q[:, 1]).mean()), "z": float(q[:, 2].mean())}
     return {"steps": steps, "psi_rel_l2": float(np.linalg.norm((a[0] - b[0]).ravel()) / np.linalg.norm(b[0].ravel())),
             "energy_ref_rel_diff": (a[1] - b[1]) / abs(b[1]),
             "energy_ref_drift": {"fp64": (b[1] - b[2]) / abs(b[2]), "fp32c": (a[1] - a[2]) / abs(a[2])},
-            "linelength": {"fp64": b[3], "fp32c": a[3]}}
+            "linelength": {"fp64": b[3], "fp32c": a[3]},
+            "core_position": core_offsets(a[4], b[4], cfg.dx), "ring": ring}
 
 
 def main():

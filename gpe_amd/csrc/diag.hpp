@@ -1,6 +1,8 @@
-// diag.hpp -- vortex-line length of a resident field (linelength,
-// variables.f90:896-1162).
+// diag.hpp -- vortex diagnostics of a resident field: the vortex-line
+// length (linelength, variables.f90:896-1162) and the vortex-core points
+// (get_zeros, get_extra_zeros and get_re_im_zeros, io.f90:886-1393).
 //
+// Line length.
 // Every interior cell (corners i..i+1, j..j+1, k..k+1, 1 <= i,j,k <= n-2)
 // is independent except for one thing: the reference keeps its point arrays
 // from cell to cell, so a cell that finds only one zero point measures the
@@ -181,6 +183,153 @@ __global__ void __launch_bounds__(256)
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// ---- vortex-core points -----------------------------------------------------
+// One thread per interior lower corner (i, j, k), 1 <= i,j,k <= n-2, as the
+// reference's loops; every test of one cell is independent of the others,
+// so hits are appended in any order (one atomic per point) and the host
+// sorts them by (section, cell) into the reference's file order.
+//
+// which = 0 (the zeros file): sections 0..2 are the edges in x, y, z on
+// which Re and Im both change sign, or vanish at the first corner
+// (io.f90:924-931), the point being where the linear interpolant of Re
+// vanishes; sections 3..5 are the xy, xz and yz faces around which Re and Im
+// change sign on alternate edges (io.f90:1039-1098 and its xz/yz copies),
+// the point being the crossing of the line through the two crossings of one
+// component with the line through the two crossings of the other, formed
+// with the reference's slopes.  which = 1 / 2 (re_zeros / im_zeros): edges
+// on which that component vanishes at the first corner or changes sign; in z
+// the reference joins the two tests with .and. (io.f90:1304, 1375), kept as
+// written.  Arithmetic in T, one rounding per operation, in the reference's
+// order (the translation unit is built without contraction).
+
+template <typename T>
+__device__ __forceinline__ T zr_interp(T w0, T w1, T a0, T a1, T d) {
+  return -(w1 * a0 / d) + w0 * a1 / d;
+}
+
+template <typename T>
+__device__ __forceinline__ bool zr_edge(T r0, T i0, T r1, T i1) {
+  return (r0 == (T)0 && i0 == (T)0) || (r0 == (T)0 && i0 * i1 < (T)0) || (i0 == (T)0 && r0 * r1 < (T)0) ||
+         (r0 * r1 < (T)0 && i0 * i1 < (T)0);
+}
+
+// corners u[1..4], v[1..4] around a face (a0,b0), (a1,b0), (a1,b1), (a0,b1):
+// u changes sign on edges 1-2 and 3-4 only, v on 2-3 and 4-1 only
+template <typename T>
+__device__ __forceinline__ bool zr_face_pattern(const T *u, const T *v) {
+  return (u[1] * u[2] < (T)0 && v[1] * v[2] >= (T)0) && (v[2] * v[3] < (T)0 && u[2] * u[3] >= (T)0) &&
+         (u[3] * u[4] < (T)0 && v[3] * v[4] >= (T)0) && (v[4] * v[1] < (T)0 && u[4] * u[1] >= (T)0);
+}
+
+template <typename T>
+__device__ __forceinline__ void zr_face_point(const T *u, const T *v, T a0, T a1, T b0, T b1, T &ap, T &bp) {
+  const T z1 = zr_interp<T>(u[1], u[2], a0, a1, u[1] - u[2]);
+  const T z2 = zr_interp<T>(v[2], v[3], b0, b1, v[2] - v[3]);
+  const T z3 = zr_interp<T>(u[3], u[4], a1, a0, u[3] - u[4]);
+  const T z4 = zr_interp<T>(v[4], v[1], b1, b0, v[4] - v[1]);
+  const T m1 = (b0 - b1) / (z1 - z3);
+  const T m2 = (z2 - z4) / (a1 - a0);
+  const T a = (z4 - a0 * m2 - b0 + z1 * m1) / (m1 - m2);
+  ap = a;
+  bp = a * m1 + b0 - z1 * m1;
+}
+
+template <typename S, typename T>
+__global__ void __launch_bounds__(256)
+    zeros_cells(const S *__restrict__ f, const S *__restrict__ lo, int nx, int ny, int nz, double xl, double yl,
+                double zl, double dxd, double dyd, double dzd, int which, ZeroPoint *__restrict__ out,
+                int *__restrict__ count, int cap) {
+  const long long cx = nx - 2, cy = ny - 2, cz = nz - 2;
+  const long long ncell = cx * cy * cz;
+  const T txl = (T)xl, tyl = (T)yl, tzl = (T)zl, dx = (T)dxd, dy = (T)dyd, dz = (T)dzd;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+       c += (long long)gridDim.x * blockDim.x) {
+    const int i = 1 + (int)(c % cx);
+    const long long r = c / cx;
+    const int j = 1 + (int)(r % cy);
+    const int k = 1 + (int)(r / cy);
+    // corners (di, dj, dk) = bits (0, 1, 2) of q
+    T re[8], im[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const size_t s = ((size_t)(k + ((q >> 2) & 1)) * ny + (j + ((q >> 1) & 1))) * nx + (i + (q & 1));
+      if (lo) {
+        re[q] = (T)((double)f[2 * s] + (double)lo[2 * s]);
+        im[q] = (T)((double)f[2 * s + 1] + (double)lo[2 * s + 1]);
+      } else {
+        re[q] = (T)f[2 * s];
+        im[q] = (T)f[2 * s + 1];
+      }
+    }
+    // x(i) = xl + real(i)*dx (ic.f90:53-66)
+    const T x0 = txl + (T)i * dx, x1 = txl + (T)(i + 1) * dx;
+    const T y0 = tyl + (T)j * dy, y1 = tyl + (T)(j + 1) * dy;
+    const T z0 = tzl + (T)k * dz, z1 = tzl + (T)(k + 1) * dz;
+    const long long cell = ((long long)k * ny + j) * nx + i;
+    const int nsec = which == 0 ? 6 : 3;
+    for (int sec = 0; sec < nsec; ++sec) {
+      T px = x0, py = y0, pz = z0;
+      bool hit = false;
+      if (sec < 3) {
+        const int q1 = 1 << sec;  // the edge's far corner: +x, +y or +z
+        T w0, w1;
+        bool cond;
+        if (which == 0) {
+          w0 = re[0];
+          w1 = re[q1];
+          cond = zr_edge<T>(re[0], im[0], re[q1], im[q1]);
+        } else {
+          w0 = which == 1 ? re[0] : im[0];
+          w1 = which == 1 ? re[q1] : im[q1];
+          cond = sec < 2 ? (w0 == (T)0 || w0 * w1 < (T)0) : (w0 == (T)0 && w0 * w1 < (T)0);
+        }
+        if (cond) {
+          const T d = w0 - w1;
+          if (d != (T)0) {
+            hit = true;
+            if (sec == 0) px = zr_interp<T>(w0, w1, x0, x1, d);
+            else if (sec == 1) py = zr_interp<T>(w0, w1, y0, y1, d);
+            else pz = zr_interp<T>(w0, w1, z0, z1, d);
+          }
+        }
+      } else {
+        // c1..c4 as corner indices, and the face axes (a, b)
+        int c2, c3, c4;
+        T a0, a1, b0, b1;
+        if (sec == 3) {         // i,j,k --> i+1,j,k --> i+1,j+1,k --> i,j+1,k
+          c2 = 1; c3 = 3; c4 = 2;
+          a0 = x0; a1 = x1; b0 = y0; b1 = y1;
+        } else if (sec == 4) {  // i,j,k --> i+1,j,k --> i+1,j,k+1 --> i,j,k+1
+          c2 = 1; c3 = 5; c4 = 4;
+          a0 = x0; a1 = x1; b0 = z0; b1 = z1;
+        } else {                // i,j,k --> i,j,k+1 --> i,j+1,k+1 --> i,j+1,k
+          c2 = 4; c3 = 6; c4 = 2;
+          a0 = z0; a1 = z1; b0 = y0; b1 = y1;
+        }
+        const T u[5] = {(T)0, re[0], re[c2], re[c3], re[c4]};
+        const T v[5] = {(T)0, im[0], im[c2], im[c3], im[c4]};
+        T ap = 0, bp = 0;
+        if (zr_face_pattern<T>(u, v)) {
+          zr_face_point<T>(u, v, a0, a1, b0, b1, ap, bp);
+          hit = true;
+        } else if (zr_face_pattern<T>(v, u)) {
+          zr_face_point<T>(v, u, a0, a1, b0, b1, ap, bp);
+          hit = true;
+        }
+        if (hit) {
+          if (sec == 3) { px = ap; py = bp; }
+          else if (sec == 4) { px = ap; pz = bp; }
+          else { pz = ap; py = bp; }
+        }
+      }
+      if (hit) {
+        const int w = atomicAdd(count, 1);
+        if (w < cap) out[w] = ZeroPoint{sec, 0, cell, (double)px, (double)py, (double)pz};
+      }
+    }
+  }
 }
 
 }  // namespace gpe

@@ -130,6 +130,8 @@ struct gpe_ctx {
   // line-length point lists (grown on demand)
   void *ll_buf = nullptr;
   int ll_cap = 0;
+  void *zr_buf = nullptr;  // zeros pass: cap points + a counter
+  int zr_cap = 0;
   double prof_ms[GPE_PROF_KINDS] = {};
 };
 
@@ -480,7 +482,7 @@ int solver_call(gpe_ctx *c, double *hist_slot) {
 
 void free_all(gpe_ctx *c) {
   void *ptrs[] = {c->psi, c->psi2, c->lo, c->lo2, c->k[0], c->k[1], c->k[2], c->k[3], c->k[4], c->k[5],
-                  c->acc, c->scale, c->part, c->dscal, c->hist, c->ll_buf};
+                  c->acc, c->scale, c->part, c->dscal, c->hist, c->ll_buf, c->zr_buf};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->hscal) hipHostFree(c->hscal);
@@ -793,6 +795,50 @@ int gpe_sim_linelength(gpe_ctx *c, double *length) {
   for (double v : part) sum += v;
   sum += c->prec == GPE_FP32 ? ll_singles<float>(w, sg) : ll_singles<double>(w, sg);
   *length = c->prec == GPE_FP32 ? (double)((float)sum * (float)c->P.dx) : sum * c->P.dx;
+  return GPE_OK;
+}
+
+int gpe_sim_zeros(gpe_ctx *c, int32_t which, gpe_zero_point *out, int64_t cap, int64_t *count) {
+  if (!c || !count) return fail(GPE_EINVAL, "NULL argument");
+  if (which < 0 || which > 2) return fail(GPE_EINVAL, "which must be 0 (zeros), 1 (re_zeros) or 2 (im_zeros)");
+  if (cap < 0 || (cap > 0 && !out)) return fail(GPE_EINVAL, "bad output buffer");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  static_assert(sizeof(gpe_zero_point) == sizeof(ZeroPoint), "gpe_zero_point layout");
+  const long long ncell = (long long)(c->P.nx - 2) * (c->P.ny - 2) * (c->P.nz - 2);
+  const int nb = (int)std::max<long long>(1, std::min<long long>((ncell + 255) / 256, 4096));
+  const double xyzl[3] = {c->P.xl, c->P.yl, c->P.zl}, dxyz[3] = {c->P.dx, c->P.dy, c->P.dz};
+  int cnt = 0;
+  for (;;) {
+    if (!c->zr_buf) {
+      const int zc = c->zr_cap > 0 ? c->zr_cap : (1 << 16);
+      if (hipMalloc(&c->zr_buf, sizeof(ZeroPoint) * (size_t)zc + 256) != hipSuccess)
+        return fail(GPE_ENOMEM, "zero list of %d points", zc);
+      c->zr_cap = zc;
+    }
+    ZeroPoint *pts = (ZeroPoint *)c->zr_buf;
+    int *counter = (int *)(pts + c->zr_cap);
+    HIPCHK(hipMemsetAsync(counter, 0, sizeof(int), c->stream));
+    hipError_t e = launch_zeros(c->prec, c->psi, c->comp ? c->lo : nullptr, c->P.nx, c->P.ny, c->P.nz, xyzl, dxyz,
+                                which, pts, counter, c->zr_cap, nb, c->stream);
+    if (e != hipSuccess) return fail(GPE_EHIP, "zeros launch: %s", hipGetErrorString(e));
+    HIPCHK(hipMemcpyAsync(&cnt, counter, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (cnt <= c->zr_cap) break;
+    // the list overflowed: grow it and run the pass again
+    if (cnt > (1 << 28)) return fail(GPE_ENOMEM, "%d zero points", cnt);
+    hipFree(c->zr_buf);
+    c->zr_buf = nullptr;
+    c->zr_cap = cnt + cnt / 4;
+  }
+  std::vector<ZeroPoint> h(cnt);
+  if (cnt) HIPCHK(hipMemcpy(h.data(), c->zr_buf, sizeof(ZeroPoint) * (size_t)cnt, hipMemcpyDeviceToHost));
+  // the reference's file order: section, then k, j, i
+  std::sort(h.begin(), h.end(), [](const ZeroPoint &a, const ZeroPoint &b) {
+    return a.section != b.section ? a.section < b.section : a.cell < b.cell;
+  });
+  const int64_t nout = std::min<int64_t>(cap, cnt);
+  if (nout) std::memcpy(out, h.data(), sizeof(ZeroPoint) * (size_t)nout);
+  *count = cnt;
   return GPE_OK;
 }
 

@@ -97,6 +97,25 @@ hipError_t launch_linelength(int prec, const void *psi, const void *lo, int nx, 
   return hipGetLastError();
 }
 
+hipError_t launch_zeros(int prec, const void *psi, const void *lo, int nx, int ny, int nz, const double *xyzl,
+                        const double *dxyz, int which, ZeroPoint *out, int *count, int cap, int nblocks,
+                        hipStream_t s) {
+  if (nx < 3 || ny < 3 || nz < 3) return hipSuccess;  // no interior cell
+  if (prec == 1)
+    hipLaunchKernelGGL((zeros_cells<float, float>), dim3(nblocks), dim3(256), 0, s, (const float *)psi,
+                       (const float *)nullptr, nx, ny, nz, xyzl[0], xyzl[1], xyzl[2], dxyz[0], dxyz[1], dxyz[2],
+                       which, out, count, cap);
+  else if (prec == 2)
+    hipLaunchKernelGGL((zeros_cells<float, double>), dim3(nblocks), dim3(256), 0, s, (const float *)psi,
+                       (const float *)lo, nx, ny, nz, xyzl[0], xyzl[1], xyzl[2], dxyz[0], dxyz[1], dxyz[2], which,
+                       out, count, cap);
+  else
+    hipLaunchKernelGGL((zeros_cells<double, double>), dim3(nblocks), dim3(256), 0, s, (const double *)psi,
+                       (const double *)nullptr, nx, ny, nz, xyzl[0], xyzl[1], xyzl[2], dxyz[0], dxyz[1], dxyz[2],
+                       which, out, count, cap);
+  return hipGetLastError();
+}
+
 hipError_t launch_reduce(const double *pn, const double *pe, const double *pen, int n, double scale_norm,
                          double eps, double scale_energy, double *out, double *hist, hipStream_t s) {
   hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(256), 0, s, pn, pe, pen, n, scale_norm, eps, scale_energy,

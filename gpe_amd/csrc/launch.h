@@ -85,6 +85,12 @@ hipError_t launch_linelength(int prec, const void *psi, const void *lo, int nx, 
                              int nblocks, LinePoint *writers, LinePoint *singles, int *counts, int cap,
                              hipStream_t s);
 
+// Vortex-core points (diag.hpp): which = 0 zeros, 1 re_zeros, 2 im_zeros;
+// appends up to cap points to out and counts all of them in *count.
+hipError_t launch_zeros(int prec, const void *psi, const void *lo, int nx, int ny, int nz, const double *xyzl,
+                        const double *dxyz, int which, ZeroPoint *out, int *count, int cap, int nblocks,
+                        hipStream_t s);
+
 // HBM read / write / copy bandwidth (GB/s) of float4 arrays of `bytes`.
 hipError_t probe_bandwidth(size_t bytes, int reps, double *gbs);
 

@@ -115,4 +115,13 @@ struct LinePoint {
   double x, y, z;
 };
 
+// A vortex-core point (zeros pass, see diag.hpp); the layout of
+// gpe_zero_point in include/gpe_hip.h.
+struct ZeroPoint {
+  int section;     // section of the reference's output file, from 0
+  int reserved;
+  long long cell;  // i + nx*(j + ny*k) of the lower corner
+  double x, y, z;
+};
+
 }  // namespace gpe

@@ -160,6 +160,31 @@ class Simulation:
         """Total vortex-line length (linelength, variables.f90:896-1162)."""
         return self._scalar(self._L.gpe_sim_linelength)
 
+    def zeros(self, which="zeros"):
+        """Vortex-core points of the resident field as the reference's
+        zeros files hold them (io.f90:886-1393): which = "zeros"
+        (get_zeros + get_extra_zeros, six sections), "re_zeros" or
+        "im_zeros" (get_re_im_zeros, three sections).  A structured array
+        (section, cell, x, y, z) in file order."""
+        w = _lib.ZERO_FILES[which]
+        n = ctypes.c_int64()
+        _lib.check(self._L.gpe_sim_zeros(self._ctx, w, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=_lib.ZERO_DTYPE)
+        m = ctypes.c_int64()
+        _lib.check(self._L.gpe_sim_zeros(self._ctx, w, out.ctypes.data_as(ctypes.c_void_p), n.value,
+                                         ctypes.byref(m)))
+        if m.value != n.value:
+            raise RuntimeError("zero count changed between calls: %d, %d" % (n.value, m.value))
+        return out[:n.value]
+
+    def save_zeros(self, proc_dir):
+        """zeros<p>.dat, re_zeros<p>.dat and im_zeros<p>.dat of the resident
+        field in the reference's text format (io.f90:886-1393); returns the
+        three paths."""
+        from .state_io import write_zeros
+        p = self.state()["p"]
+        return [write_zeros(proc_dir, w, p, self.zeros(w), self.cfg.nz) for w in ("zeros", "re_zeros", "im_zeros")]
+
     def renormalise(self, nn, norm):
         _lib.check(self._L.gpe_sim_renormalise(self._ctx, float(nn), float(norm)))
 

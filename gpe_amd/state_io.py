@@ -7,6 +7,8 @@ and state_restart (ic.f90:117-191) reads the same layout from
 end_state####.dat.  pr is double or single precision (parameters.in:22-23).
 Fields here are numpy arrays of shape (nz, ny, nx).
 """
+import os
+
 import numpy as np
 
 
@@ -65,7 +67,6 @@ def write_dens(proc_dir, psi, cfg, p, t=0.0, im_t=0.0):
     int32 js, je, ks, ke; complex(pr) psi(0:nx1, js:je, ks:ke); real(pr)
     x(0:nx1), y(0:ny1), z(0:nz1).  Appends p to p_saved.dat ('(i10)') in the
     parent directory, as rank 0 does.  Returns the file path."""
-    import os
     psi = np.ascontiguousarray(psi)
     rt = np.float64 if psi.dtype == np.complex128 else np.float32
     nz, ny, nx = psi.shape
@@ -96,3 +97,68 @@ def read_dens(path, single=False):
         y = np.frombuffer(fh.read(ny * np.dtype(rt).itemsize), dtype=rt)
         z = np.frombuffer(fh.read(nz * np.dtype(rt).itemsize), dtype=rt)
     return psi.copy(), tt, (nyp, nzp), (js, je, ks, ke), x.copy(), y.copy(), z.copy()
+
+
+# Section headers of the zeros files, as get_zeros / get_extra_zeros
+# (io.f90:919, 945, 971, 1037, 1102, 1165) and get_re_im_zeros
+# (io.f90:1263, 1282, 1302; the im file repeats them) write them with
+# list-directed output, which puts one blank in front.
+ZERO_HEADERS = {
+    "zeros": ["# i,j,k --> i+1,j,k", "# i,j,k --> i,j+1,k", "# i,j,k --> i,j,k+1",
+              "# i,j,k --> i+1,j,k --> i+1,j+1,k --> i,j+1,k", "# i,j,k --> i+1,j,k --> i+1,j,k+1 --> i,j,k+1",
+              "# i,j,k --> i,j,k+1 --> i,j+1,k+1 --> i,j+1,k"],
+    "re_zeros": ["# i,j,k --> i+1,j,k", "# i,j,k --> i,j+1,k", "# i,j,k --> i,j,k+1"],
+    "im_zeros": ["# i,j,k --> i+1,j,k", "# i,j,k --> i,j+1,k", "# i,j,k --> i,j,k+1"],
+}
+
+
+def fortran_e(v, w=17, d=9):
+    """One value in the Fortran edit descriptor Ew.d (the reference writes
+    zero points with '(3e17.9)'): 0.<d digits>E+xx, right-justified."""
+    v = float(v)
+    if v != v:
+        s = "NaN"
+    elif v in (float("inf"), float("-inf")):
+        s = "Infinity" if v > 0 else "-Infinity"
+    elif v == 0.0:
+        s = ("-" if np.signbit(v) else "") + "0." + "0" * d + "E+00"
+    else:
+        m = "%.*e" % (d - 1, abs(v))
+        mant, ex = m.split("e")
+        digits = mant.replace(".", "")
+        e = int(ex) + 1
+        s = ("-" if v < 0 else "") + "0." + digits + ("E%+03d" % e if abs(e) <= 99 else "%+04d" % e)
+    return s.rjust(w)
+
+
+def write_zeros(proc_dir, which, p, points, nz):
+    """proc_dir/<which><itos(p)>.dat with the points of a zeros pass (a
+    structured array with fields section, x, y, z in file order).  The z
+    sections are written only for a 3-D grid (z_start /= z_end,
+    io.f90:970, 1101, 1301)."""
+    os.makedirs(proc_dir, exist_ok=True)
+    path = os.path.join(proc_dir, "%s%s.dat" % (which, itos(p)))
+    heads = ZERO_HEADERS[which]
+    sec = np.asarray(points["section"])
+    with open(path, "w") as fh:
+        for s, h in enumerate(heads):
+            if nz <= 1 and ((which == "zeros" and s in (2, 4, 5)) or (which != "zeros" and s == 2)):
+                continue
+            fh.write(" " + h + "\n")
+            for q in np.flatnonzero(sec == s):
+                fh.write(fortran_e(points["x"][q]) + fortran_e(points["y"][q]) + fortran_e(points["z"][q]) + "\n")
+    return path
+
+
+def read_zeros(path):
+    """Sections of a zeros file: a list of (n, 3) float arrays."""
+    out = []
+    cur = None
+    with open(path) as fh:
+        for line in fh:
+            if line.strip().startswith("#"):
+                cur = []
+                out.append(cur)
+            elif line.strip():
+                cur.append([float(line[q:q + 17].replace("D", "E")) for q in (0, 17, 34)])
+    return [np.array(s, dtype=np.float64).reshape(-1, 3) for s in out]

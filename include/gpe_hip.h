@@ -122,6 +122,23 @@ int gpe_sim_mass(gpe_ctx *ctx, double *mass);     /* mass */
  * reference's cell-by-cell zero tracing (including its reuse of the previous
  * cell's second point in cells that find a single point). */
 int gpe_sim_linelength(gpe_ctx *ctx, double *length);
+/* Vortex-core points of the resident field, as the reference's zeros files
+ * hold them: which = 0 is zeros<p>.dat (get_zeros + get_extra_zeros,
+ * io.f90:886-1236; sections 0..2 = edges in x, y, z on which Re and Im both
+ * change sign, 3..5 = xy, xz, yz faces around which their zero lines
+ * cross), 1 is re_zeros<p>.dat and 2 is im_zeros<p>.dat (get_re_im_zeros,
+ * io.f90:1237-1393; sections 0..2 = edges in x, y, z).  Points come in the
+ * file's order (section, then k, j, i of the cell's lower corner), in the
+ * precision of the storage form (FP32: single-precision arithmetic).
+ * Writes min(count, cap) points to out and the number found to *count;
+ * call with cap = 0 to size the buffer. */
+typedef struct gpe_zero_point {
+  int32_t section;   /* section of the file, from 0 */
+  int32_t reserved;
+  int64_t cell;      /* i + nx*(j + ny*k) of the lower corner */
+  double x, y, z;    /* the point */
+} gpe_zero_point;
+int gpe_sim_zeros(gpe_ctx *ctx, int32_t which, gpe_zero_point *out, int64_t cap, int64_t *count);
 /* renormalise (variables.f90:715-726): psi = sqrt(nn)*psi/sqrt(norm). */
 int gpe_sim_renormalise(gpe_ctx *ctx, double nn, double norm);
 /* f(psi) of the resident field into a host buffer (get_rhs, solve.f90:382). */

@@ -24,6 +24,8 @@
  *   energy / mass     variables.f90:580-628
  *   renormalise       variables.f90:715-726
  *   linelength        variables.f90:896-1162
+ *   zeros             io.f90:886-1393 (get_zeros, get_extra_zeros,
+ *                     get_re_im_zeros)
  *
  * Complex arithmetic follows what this image's flang (the compiler the
  * reference is built with here) emits for the reference's expressions,
@@ -827,4 +829,154 @@ double FN(linelength)(const orc_params *p, const REAL *f) {
         }
       }
   return (double)(L * (REAL)p->dx);
+}
+
+/* ---- vortex cores: get_zeros, get_extra_zeros, get_re_im_zeros -----------
+ * (io.f90:886-1236 and 1237-1393.)  Every interior lower corner (i, j, k)
+ * with 1 <= i <= nx-2, 1 <= j <= ny-2, 1 <= k <= nz-2 is tested; output is
+ * in the reference's file order: section by section, k slowest, i fastest.
+ *
+ * which = 0, the zeros<p>.dat file, six sections:
+ *   0..2  edges (i,j,k)->(i+1,j,k), ->(i,j+1,k), ->(i,j,k+1) on which Re and
+ *         Im both change sign (or vanish at the first corner, io.f90:924-931);
+ *         the point is where the linear interpolant of Re vanishes; an edge
+ *         with equal Re at both ends is skipped;
+ *   3..5  faces xy, xz, yz (corners in the order the section header lists)
+ *         around which Re and Im change sign on alternate edges, one of two
+ *         patterns (io.f90:1039-1098 and the xz/yz copies); the point is the
+ *         intersection of the line through the two Re crossings with the
+ *         line through the two Im crossings, formed with the reference's
+ *         slopes m(1), m(2) (no guard against a zero denominator).
+ * which = 1 / 2, the re_zeros / im_zeros files, three sections: edges in x,
+ * y, z on which that component vanishes at the first corner or changes
+ * sign.  In z the reference joins the two tests with .and. (io.f90:1304 and
+ * 1375), which no finite value satisfies, so that section stays empty; this
+ * restatement keeps the test as written.
+ *
+ * Each point carries its section and the linear index i + nx*(j + ny*k) of
+ * the lower corner.  Returns the number of points; the first `cap` are
+ * written. */
+static int zr_edge(REAL r0, REAL i0, REAL r1, REAL i1) {
+  return (r0 == (REAL)0.0 && i0 == (REAL)0.0) || (r0 == (REAL)0.0 && i0 * i1 < (REAL)0.0) ||
+         (i0 == (REAL)0.0 && r0 * r1 < (REAL)0.0) || (r0 * r1 < (REAL)0.0 && i0 * i1 < (REAL)0.0);
+}
+
+/* zero = -w1*a0/d + w0*a1/d with d = w0 - w1 (io.f90:933-935) */
+static REAL zr_interp(REAL w0, REAL w1, REAL a0, REAL a1, REAL d) { return -(w1 * a0 / d) + w0 * a1 / d; }
+
+/* Face with corners c1 = (a0,b0), c2 = (a1,b0), c3 = (a1,b1), c4 = (a0,b1):
+ * u changes sign on edges 1-2 and 3-4 (v does not), v on edges 2-3 and
+ * 4-1 (u does not).  io.f90:1039-1065 with u = Re, v = Im; 1066-1092 with
+ * u = Im, v = Re. */
+static int zr_face_pattern(const REAL *u, const REAL *v) {
+  return (u[1] * u[2] < (REAL)0.0 && v[1] * v[2] >= (REAL)0.0) && (v[2] * v[3] < (REAL)0.0 && u[2] * u[3] >= (REAL)0.0) &&
+         (u[3] * u[4] < (REAL)0.0 && v[3] * v[4] >= (REAL)0.0) && (v[4] * v[1] < (REAL)0.0 && u[4] * u[1] >= (REAL)0.0);
+}
+
+static void zr_face_point(const REAL *u, const REAL *v, REAL a0, REAL a1, REAL b0, REAL b1, REAL *ap, REAL *bp) {
+  const REAL z1 = zr_interp(u[1], u[2], a0, a1, u[1] - u[2]);
+  const REAL z2 = zr_interp(v[2], v[3], b0, b1, v[2] - v[3]);
+  const REAL z3 = zr_interp(u[3], u[4], a1, a0, u[3] - u[4]);
+  const REAL z4 = zr_interp(v[4], v[1], b1, b0, v[4] - v[1]);
+  const REAL m1 = (b0 - b1) / (z1 - z3);
+  const REAL m2 = (z2 - z4) / (a1 - a0);
+  const REAL a = (z4 - a0 * m2 - b0 + z1 * m1) / (m1 - m2);
+  *ap = a;
+  *bp = a * m1 + b0 - z1 * m1;
+}
+
+long FN(zeros)(const orc_params *p, const REAL *f, int which, orc_zero *out, long cap) {
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  ctx c;
+  ctx_init(&c, p);
+  const int nsec = which == 0 ? 6 : 3;
+  long n = 0;
+#define ZRE(ii, jj, kk) f[2 * (((long)(kk) * ny + (jj)) * nx + (ii))]
+#define ZIM(ii, jj, kk) f[2 * (((long)(kk) * ny + (jj)) * nx + (ii)) + 1]
+  for (int sec = 0; sec < nsec; sec++)
+    for (int k = 1; k <= nz - 2; k++)
+      for (int j = 1; j <= ny - 2; j++)
+        for (int i = 1; i <= nx - 2; i++) {
+          REAL px = c.x[i], py = c.y[j], pz = c.z[k];
+          int hit = 0;
+          if (which == 0 && sec < 3) {
+            const int di = sec == 0, dj = sec == 1, dk = sec == 2;
+            const REAL r0 = ZRE(i, j, k), i0 = ZIM(i, j, k);
+            const REAL r1 = ZRE(i + di, j + dj, k + dk), i1 = ZIM(i + di, j + dj, k + dk);
+            if (zr_edge(r0, i0, r1, i1)) {
+              const REAL d = r0 - r1;
+              if (d != (REAL)0.0) {
+                hit = 1;
+                if (sec == 0) px = zr_interp(r0, r1, c.x[i], c.x[i + 1], d);
+                if (sec == 1) py = zr_interp(r0, r1, c.y[j], c.y[j + 1], d);
+                if (sec == 2) pz = zr_interp(r0, r1, c.z[k], c.z[k + 1], d);
+              }
+            }
+          } else if (which == 0) {
+            /* corner offsets of c1..c4 and the face axes (a, b) */
+            int o[5][3];
+            REAL a0, a1, b0, b1;
+            if (sec == 3) {        /* i,j,k --> i+1,j,k --> i+1,j+1,k --> i,j+1,k */
+              int t[5][3] = {{0, 0, 0}, {0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}};
+              memcpy(o, t, sizeof t);
+              a0 = c.x[i]; a1 = c.x[i + 1]; b0 = c.y[j]; b1 = c.y[j + 1];
+            } else if (sec == 4) { /* i,j,k --> i+1,j,k --> i+1,j,k+1 --> i,j,k+1 */
+              int t[5][3] = {{0, 0, 0}, {0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}};
+              memcpy(o, t, sizeof t);
+              a0 = c.x[i]; a1 = c.x[i + 1]; b0 = c.z[k]; b1 = c.z[k + 1];
+            } else {               /* i,j,k --> i,j,k+1 --> i,j+1,k+1 --> i,j+1,k */
+              int t[5][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}};
+              memcpy(o, t, sizeof t);
+              a0 = c.z[k]; a1 = c.z[k + 1]; b0 = c.y[j]; b1 = c.y[j + 1];
+            }
+            REAL re[5], im[5];
+            for (int q = 1; q <= 4; q++) {
+              re[q] = ZRE(i + o[q][0], j + o[q][1], k + o[q][2]);
+              im[q] = ZIM(i + o[q][0], j + o[q][1], k + o[q][2]);
+            }
+            REAL ap = 0, bp = 0;
+            if (zr_face_pattern(re, im)) {
+              zr_face_point(re, im, a0, a1, b0, b1, &ap, &bp);
+              hit = 1;
+            } else if (zr_face_pattern(im, re)) {
+              zr_face_point(im, re, a0, a1, b0, b1, &ap, &bp);
+              hit = 1;
+            }
+            if (hit) {
+              if (sec == 3) { px = ap; py = bp; }
+              else if (sec == 4) { px = ap; pz = bp; }
+              else { pz = ap; py = bp; }
+            }
+          } else {
+            const int di = sec == 0, dj = sec == 1, dk = sec == 2;
+            const REAL w0 = which == 1 ? ZRE(i, j, k) : ZIM(i, j, k);
+            const REAL w1 = which == 1 ? ZRE(i + di, j + dj, k + dk) : ZIM(i + di, j + dj, k + dk);
+            const int cond = sec < 2 ? (w0 == (REAL)0.0 || w0 * w1 < (REAL)0.0)
+                                     : (w0 == (REAL)0.0 && w0 * w1 < (REAL)0.0);
+            if (cond) {
+              const REAL d = w0 - w1;
+              if (d != (REAL)0.0) {
+                hit = 1;
+                if (sec == 0) px = zr_interp(w0, w1, c.x[i], c.x[i + 1], d);
+                if (sec == 1) py = zr_interp(w0, w1, c.y[j], c.y[j + 1], d);
+                if (sec == 2) pz = zr_interp(w0, w1, c.z[k], c.z[k + 1], d);
+              }
+            }
+          }
+          if (hit) {
+            if (n < cap) {
+              out[n].section = sec;
+              out[n].reserved = 0;
+              out[n].cell = ((long long)k * ny + j) * nx + i;
+              out[n].x = (double)px;
+              out[n].y = (double)py;
+              out[n].z = (double)pz;
+            }
+            n++;
+          }
+        }
+#undef ZRE
+#undef ZIM
+  ctx_free(&c);
+  return n;
 }

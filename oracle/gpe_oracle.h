@@ -43,6 +43,14 @@ typedef struct orc_state {
   int p;                /* step counter (gpe.f90:279) */
 } orc_state;
 
+/* One vortex-core point (get_zeros / get_extra_zeros / get_re_im_zeros). */
+typedef struct orc_zero {
+  int section;        /* section of the reference's output file, from 0 */
+  int reserved;
+  long long cell;     /* i + nx*(j + ny*k) of the lower corner */
+  double x, y, z;
+} orc_zero;
+
 #define ORC_DECL(P, REAL)                                                     \
   void P##_grid(const orc_params *p, REAL *x, REAL *y, REAL *z);             \
   void P##_laplacian(const orc_params *p, const REAL *f, REAL *lap);        \
@@ -56,7 +64,9 @@ typedef struct orc_state {
   double P##_energy(const orc_params *p, const REAL *f);                    \
   double P##_mass(const orc_params *p, const REAL *f);                      \
   void P##_renormalise(const orc_params *p, REAL *f, double nn, double norm); \
-  double P##_linelength(const orc_params *p, const REAL *f);
+  double P##_linelength(const orc_params *p, const REAL *f);                \
+  long P##_zeros(const orc_params *p, const REAL *f, int which,             \
+                 orc_zero *out, long cap);
 
 ORC_DECL(orc64, double)
 ORC_DECL(orc32, float)

@@ -65,6 +65,7 @@ def lib():
             for nm in ("_norm", "_energy", "_mass", "_linelength"):
                 f = getattr(_LIB, pfx + nm); f.argtypes = [P, vp]; f.restype = D
             f = getattr(_LIB, pfx + "_renormalise"); f.argtypes = [P, vp, D, D]; f.restype = None
+            f = getattr(_LIB, pfx + "_zeros"); f.argtypes = [P, vp, I, vp, ctypes.c_long]; f.restype = ctypes.c_long
     return _LIB
 
 
@@ -175,6 +176,24 @@ def renormalise(cfg, psi, nn, nrm):
     out = np.ascontiguousarray(psi).copy()
     getattr(lib(), _pfx(out) + "_renormalise")(ctypes.byref(params(cfg, out.shape)), _ptr(out), float(nn), float(nrm))
     return out
+
+
+ZERO_DTYPE = np.dtype([("section", np.int32), ("reserved", np.int32), ("cell", np.int64), ("x", np.float64),
+                       ("y", np.float64), ("z", np.float64)])
+
+
+def zeros(cfg, psi, which=0):
+    """Vortex-core points of the field (io.f90:886-1393): which = 0 for the
+    zeros file (get_zeros + get_extra_zeros, sections 0..5), 1 for re_zeros,
+    2 for im_zeros (sections 0..2).  A structured array in file order."""
+    psi = np.ascontiguousarray(psi)
+    f = getattr(lib(), _pfx(psi) + "_zeros")
+    prm = params(cfg, psi.shape)
+    n = f(ctypes.byref(prm), _ptr(psi), int(which), None, 0)
+    out = np.zeros(max(n, 1), dtype=ZERO_DTYPE)
+    n2 = f(ctypes.byref(prm), _ptr(psi), int(which), out.ctypes.data_as(ctypes.c_void_p), n)
+    assert n2 == n
+    return out[:n]
 
 
 def linelength(cfg, psi):

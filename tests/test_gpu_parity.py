@@ -258,9 +258,23 @@ def test_config0_ring_rk4_1000_steps(gpu, oracle):
     with Simulation(cfg.with_(precision="fp32c"), psi) as sim:
         sim.advance(1000)
         got32 = sim.download(np.complex128)
+        z32 = sim.zeros()
     err32 = rel_l2(got32, want)
     say("config 0 fp32c rel L2 = %.3e" % err32)
     assert err32 < 1e-5
+    # vortex-core positions after 1000 steps (the reference's zeros file):
+    # the GPU's FP64 field and the oracle's give the same core points to
+    # 1e-9 dx; the compensated form's cores sit within 1e-4 dx of them
+    from gpe_amd.accuracy import core_offsets
+    with Simulation(cfg, got) as sim:
+        z64 = sim.zeros()
+    zo = oracle.zeros(cfg, want, 0)
+    d64 = core_offsets(z64, zo, cfg.dx)
+    d32 = core_offsets(z32, zo, cfg.dx)
+    say("config 0 core offsets fp64 %s fp32c %s" % (d64, d32))
+    assert d64["points"][0] == d64["points"][1] > 50
+    assert d64["max_dx"] < 1e-9
+    assert d32["mean_dx"] < 1e-4 and d32["frac_over_tenth_dx"] == 0.0
 
 
 def test_config1_line_pair_256(gpu, oracle):

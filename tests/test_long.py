@@ -3,7 +3,8 @@ skipped unless GPE_LONG=1.
 
 The 256^3 line pair in a trap (dx = 0.25, 4th-order FD, reflective, RK4
 dt = 0.005) for its full 2000 steps: FP64 psi, norm and energy within
-1e-12 of the oracle."""
+1e-12 of the oracle, and the vortex-core points of the two lines within
+1e-9 dx of the oracle's."""
 import os
 import sys
 import time
@@ -50,3 +51,11 @@ def test_config1_line_pair_256_full(oracle):
     assert err < 1e-12
     assert abs(gn[-1] - wn[-1]) < 1e-12 * wn[-1]
     assert ge == pytest.approx(we, rel=1e-12)
+    # vortex-core positions of the two lines (the reference's zeros file)
+    from gpe_amd.accuracy import core_offsets
+    with Simulation(cfg, got) as sim:
+        zg = sim.zeros()
+    d = core_offsets(zg, oracle.zeros(cfg, want, 0), cfg.dx)
+    print("[result] core offsets %s" % d, flush=True)
+    assert d["points"][0] == d["points"][1] > 100
+    assert d["max_dx"] < 1e-9
