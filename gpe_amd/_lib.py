@@ -35,7 +35,8 @@ ZERO_FILES = {"zeros": 0, "re_zeros": 1, "im_zeros": 2}
 
 # Exported entry points (must match include/gpe_hip.h).
 EXPORTS = [
-    "gpe_last_error", "gpe_ctx_create", "gpe_ctx_destroy", "gpe_solver_step",
+    "gpe_last_error", "gpe_ctx_create", "gpe_ctx_destroy", "gpe_solver_step", "gpe_solver_step_ex",
+    "gpe_transfer_bytes",
     "gpe_sim_upload", "gpe_sim_download", "gpe_sim_set_state", "gpe_sim_get_state",
     "gpe_sim_advance", "gpe_sim_norm_history", "gpe_sim_norm", "gpe_sim_energy",
     "gpe_sim_mass", "gpe_sim_linelength", "gpe_sim_zeros", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
@@ -96,6 +97,8 @@ def lib():
         "gpe_ctx_create": ([P, ctypes.POINTER(vp)], ctypes.c_int),
         "gpe_ctx_destroy": ([vp], None),
         "gpe_solver_step": ([vp, vp, i32, vp, S], ctypes.c_int),
+        "gpe_solver_step_ex": ([vp, vp, i32, vp, S, i32], ctypes.c_int),
+        "gpe_transfer_bytes": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),
         "gpe_sim_upload": ([vp, vp, i32], ctypes.c_int),
         "gpe_sim_download": ([vp, vp, i32], ctypes.c_int),
         "gpe_sim_set_state": ([vp, S], ctypes.c_int),

@@ -107,6 +107,7 @@ struct gpe_ctx {
   double *hist = nullptr;
   int64_t hist_cap = 0, hist_len = 0;
   bool uploaded = false;
+  int64_t bytes_h2d = 0, bytes_d2h = 0;  // host field traffic
   gpe_step_state st{};
   StageArgs base{};
   // launch configuration per stage kind: tile variant (index into kTiles)
@@ -514,13 +515,17 @@ int alloc_field(gpe_ctx *c, void **p) {
 }
 
 int to_device(gpe_ctx *c, const void *src, int host_type, size_t src_pitch_planes, size_t row_off) {
-  return field_upload(c->prec, (size_t)c->P.nx * c->P.ny, c->P.nz, src, host_type, src_pitch_planes, row_off,
-                      c->psi, c->lo);
+  const int rc = field_upload(c->prec, (size_t)c->P.nx * c->P.ny, c->P.nz, src, host_type, src_pitch_planes,
+                              row_off, c->psi, c->lo);
+  if (rc == GPE_OK) c->bytes_h2d += (int64_t)(c->n * (host_type == GPE_HOST_C128 ? 16 : 8));
+  return rc;
 }
 
 int from_device(gpe_ctx *c, void *dst, int host_type) {
   GPE_HIPCHK(hipStreamSynchronize(c->stream));
-  return field_download(c->prec, (size_t)c->P.nx * c->P.ny, c->P.nz, c->psi, c->lo, dst, host_type);
+  const int rc = field_download(c->prec, (size_t)c->P.nx * c->P.ny, c->P.nz, c->psi, c->lo, dst, host_type);
+  if (rc == GPE_OK) c->bytes_d2h += (int64_t)(c->n * (host_type == GPE_HOST_C128 ? 16 : 8));
+  return rc;
 }
 
 }  // namespace
@@ -882,22 +887,42 @@ int gpe_sim_rhs(gpe_ctx *c, void *out, int32_t host_type) {
 }
 
 int gpe_solver_step(gpe_ctx *c, const void *var_in, int32_t halo, void *var_out, gpe_step_state *st) {
+  return gpe_solver_step_ex(c, var_in, halo, var_out, st, 0);
+}
+
+int gpe_transfer_bytes(gpe_ctx *c, int64_t *to_device, int64_t *to_host) {
+  if (!c || !to_device || !to_host) return fail(GPE_EINVAL, "NULL argument");
+  *to_device = c->bytes_h2d;
+  *to_host = c->bytes_d2h;
+  return GPE_OK;
+}
+
+int gpe_solver_step_ex(gpe_ctx *c, const void *var_in, int32_t halo, void *var_out, gpe_step_state *st,
+                       int32_t flags) {
   if (!c || !var_in || !var_out || !st) return fail(GPE_EINVAL, "NULL argument");
   if (halo < 0) return fail(GPE_EINVAL, "halo < 0");
+  if (flags & ~(GPE_STEP_IN_RESIDENT | GPE_STEP_OUT_DEFERRED)) return fail(GPE_EINVAL, "unknown step flags");
   if (c->prec == GPE_FP32C) return fail(GPE_EINVAL, "the compensated form is device-resident only");
   const int ht = c->prec == GPE_FP64 ? GPE_HOST_C128 : GPE_HOST_C64;
-  const size_t nx = c->P.nx, nyh = (size_t)c->P.ny + 2 * halo;
-  const size_t pitch = nx * nyh;                       // elements per host z-plane
-  const size_t off = (size_t)halo * nx + (size_t)halo * pitch;  // (0, js, ks)
-  HIPCHK(hipStreamSynchronize(c->stream));
-  int rc = to_device(c, var_in, ht, pitch, off);
-  if (rc) return rc;
-  c->uploaded = true;
+  int rc;
+  if (flags & GPE_STEP_IN_RESIDENT) {
+    if (!c->uploaded) return fail(GPE_ESTATE, "GPE_STEP_IN_RESIDENT before any field was uploaded");
+  } else {
+    const size_t nx = c->P.nx, nyh = (size_t)c->P.ny + 2 * halo;
+    const size_t pitch = nx * nyh;                                // elements per host z-plane
+    const size_t off = (size_t)halo * nx + (size_t)halo * pitch;  // (0, js, ks)
+    HIPCHK(hipStreamSynchronize(c->stream));
+    rc = to_device(c, var_in, ht, pitch, off);
+    if (rc) return rc;
+    c->uploaded = true;
+  }
   c->st = *st;
   rc = solver_call(c, nullptr);
   if (rc) return rc;
-  rc = from_device(c, var_out, ht);
-  if (rc) return rc;
+  if (flags & GPE_STEP_OUT_DEFERRED)
+    HIPCHK(hipStreamSynchronize(c->stream));
+  else if ((rc = from_device(c, var_out, ht)))
+    return rc;
   gpe_step_state o = c->st;
   o.p = st->p;
   *st = o;

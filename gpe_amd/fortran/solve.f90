@@ -14,13 +14,32 @@
 !
 ! One rank only (nyprocs = nzprocs = 1): boundaries are applied in the GPU
 ! kernels, so the ghost planes of var_in are not read.
+!
+! Device-authoritative mode (environment GPE_RESIDENT=1, real time only, not
+! with diagnostic): the field stays in HBM across calls.  The unchanged
+! driver copies psi%new into psi%old and calls solver() every step
+! (gpe.f90:167-176, 230), but it reads psi only for its periodic I/O
+! (gpe.f90:178-228, 234, 245-247) and at the end of the run.  So the module
+! uploads var_in once, skips the upload while var_in is the field it
+! returned, and writes var_out only when the driver will read psi before the
+! next call.  It predicts that from the driver's own schedule: the
+! save_rate test, the save_rate2/save_rate3/p_save thresholds with mirrors
+! of the driver's counters n, m, ps (gpe.f90:127-129 and their
+! increments), the end_time test of the loop, and the SAVE and RUNNING
+! files.  Two reads it cannot predict: a RUNNING file deleted after this
+! call returns (then end_state writes psi%new, io.f90:831) and a SAVE file
+! created after it (save_run, io.f90:1558).  A call of solver_sync(in_var)
+! at the top of end_state and inside save_run's `if (save3d == 1)` makes
+! those exact.  get_norm(psi%new) runs every step (gpe.f90:244); with
+! `if (solver_norm(in_var, norm)) return` at its top it takes the norm of
+! the device field instead of the stale host copy.
 module solve
   use parameters
   use iso_c_binding
   implicit none
 
   private
-  public :: solver
+  public :: solver, solver_norm, solver_sync, solver_transfer_bytes
 
   ! mirrors struct gpe_params (include/gpe_hip.h)
   type, bind(C) :: gpe_params_t
@@ -49,13 +68,46 @@ module solve
       type(gpe_step_state_t), intent(inout) :: st
       integer(c_int) :: rc
     end function gpe_solver_step
+    function gpe_solver_step_ex(ctx, vin, halo, vout, st, flags) bind(C, name='gpe_solver_step_ex') result(rc)
+      import :: c_ptr, c_int, c_int32_t, gpe_step_state_t
+      type(c_ptr), value :: ctx, vin, vout
+      integer(c_int32_t), value :: halo, flags
+      type(gpe_step_state_t), intent(inout) :: st
+      integer(c_int) :: rc
+    end function gpe_solver_step_ex
+    function gpe_sim_download(ctx, psi, host_type) bind(C, name='gpe_sim_download') result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx, psi
+      integer(c_int32_t), value :: host_type
+      integer(c_int) :: rc
+    end function gpe_sim_download
+    function gpe_sim_norm(ctx, norm) bind(C, name='gpe_sim_norm') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: norm
+      integer(c_int) :: rc
+    end function gpe_sim_norm
+    function gpe_transfer_bytes(ctx, h2d, d2h) bind(C, name='gpe_transfer_bytes') result(rc)
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: h2d, d2h
+      integer(c_int) :: rc
+    end function gpe_transfer_bytes
     function gpe_last_error() bind(C, name='gpe_last_error') result(msg)
       import :: c_ptr
       type(c_ptr) :: msg
     end function gpe_last_error
   end interface
 
+  integer(c_int32_t), parameter :: step_in_resident = 1, step_out_deferred = 2  ! include/gpe_hip.h
+
   type(c_ptr), save :: ctx = c_null_ptr
+  ! device-authoritative mode
+  logical, save :: resident = .false.   ! GPE_RESIDENT=1 at the first call
+  logical, save :: dev_holds = .false.  ! the context holds the current field
+  logical, save :: host_stale = .false. ! held_out is older than the device field
+  type(c_ptr), save :: held_out = c_null_ptr
+  integer, save :: n_io = 0, m_io = 0, ps_io = 0  ! mirrors of gpe.f90's n, m, ps
 
   contains
 
@@ -66,11 +118,36 @@ module solve
     implicit none
 
     complex (pr), dimension(0:nx1,js-2:je+2,ks-2:ke+2), intent(in), target :: var_in
-    complex (pr), dimension(0:nx1,js:je,ks:ke), intent(out), target :: var_out
+    ! inout: in device-authoritative mode var_out keeps its contents on the
+    ! steps after which the driver does not read it
+    complex (pr), dimension(0:nx1,js:je,ks:ke), intent(inout), target :: var_out
     type(gpe_step_state_t) :: st
     integer(c_int) :: rc
+    integer(c_int32_t) :: flags
+    logical :: use_res
 
     if (.not. c_associated(ctx)) call create_context()
+
+    use_res = resident .and. real_time .and. .not. diagnostic
+    flags = 0
+    if (use_res) then
+      if (.not. dev_holds) then
+        ! first resident step (or the first after imaginary time): the
+        ! driver's counters as gpe.f90:127-129 (and 258-260) set them
+        ps_io = int(t/p_save)
+        n_io = int(t/save_rate2)
+        m_io = int(t/save_rate3)
+      end if
+      ! the increments the driver made in this iteration (gpe.f90:188-228)
+      if (t+im_t >= save_rate3*m_io) m_io = m_io+1
+      if (t+im_t >= save_rate2*n_io) n_io = n_io+1
+      if (t+im_t >= p_save*ps_io) ps_io = ps_io+1
+      flags = step_out_deferred
+      if (dev_holds) flags = flags + step_in_resident
+    else
+      dev_holds = .false.
+      host_stale = .false.
+    end if
 
     st%dt_re = real(dt, c_double)
     st%dt_im = real(aimag(dt), c_double)
@@ -80,12 +157,19 @@ module solve
     st%errmax = 0.0_c_double
     st%n_rejects = 0
 
-    rc = gpe_solver_step(ctx, c_loc(var_in), 2_c_int32_t, c_loc(var_out), st)
+    rc = gpe_solver_step_ex(ctx, c_loc(var_in), 2_c_int32_t, c_loc(var_out), st, flags)
     if (rc /= 0) call emergency_stop('ERROR: '//last_error())
 
     dt = cmplx(st%dt_re, st%dt_im, pr)
     t = real(st%t, pr)
     im_t = real(st%im_t, pr)
+
+    if (use_res) then
+      dev_holds = .true.
+      held_out = c_loc(var_out)
+      host_stale = .true.
+      if (host_reads_next()) call solver_sync(var_out)
+    end if
 
     if (scheme == 'rk45') then
       errcon = (5.0_pr/safety)**(1.0_pr/dt_increase)
@@ -103,6 +187,109 @@ module solve
 
 ! ***************************************************************************
 
+  logical function host_reads_next()
+    ! Will the driver read psi before its next solver() call?  The tests of
+    ! gpe.f90's loop, evaluated with the time, dt and counters it will use.
+    use parameters
+    implicit none
+
+    logical :: file_exists
+
+    host_reads_next = .true.
+    ! the loop ends (gpe.f90:136): end_state, idl_surface, save_surface
+    if (t+im_t > end_time) return
+    ! save_norm now (245-247); save_energy, save_time, save_linelength at
+    ! the start of the next iteration (178-186)
+    if (modulo(t+im_t, abs(dt)*save_rate) < abs(dt)) return
+    ! condensed_particles, pdf, vcf (188-197)
+    if (t+im_t >= save_rate3*m_io) return
+    ! zeros, surfaces, averages (199-221) and diag (236-240)
+    if (t+im_t >= save_rate2*n_io) return
+    ! periodic end_state (224-228)
+    if (t+im_t >= p_save*ps_io) return
+    ! save_run (234, io.f90:1558-1587)
+    inquire (file='SAVE', exist=file_exists)
+    if (file_exists) return
+    ! a stop request (139-159): end_state(psi%new, 1)
+    inquire (file='RUNNING', exist=file_exists)
+    if (.not. file_exists) return
+    host_reads_next = .false.
+
+    return
+  end function host_reads_next
+
+! ***************************************************************************
+
+  subroutine solver_sync(var)
+    ! Bring var up to date if it is the field a device-authoritative step
+    ! left stale (a no-op otherwise).  For a host reader the module cannot
+    ! predict: call it at the top of end_state (io.f90:831) and inside
+    ! save_run's `if (save3d == 1)` (io.f90:1578).
+    use error
+    use parameters
+    implicit none
+
+    complex (pr), dimension(0:nx1,js:je,ks:ke), intent(inout), target :: var
+    integer(c_int) :: rc
+    integer(c_int32_t) :: ht
+
+    if (.not. host_stale) return
+    if (.not. c_associated(held_out, c_loc(var))) return
+    ht = 0
+    if (kind(var) /= kind((1.0d0, 1.0d0))) ht = 1
+    rc = gpe_sim_download(ctx, c_loc(var), ht)
+    if (rc /= 0) call emergency_stop('ERROR: '//last_error())
+    host_stale = .false.
+
+    return
+  end subroutine solver_sync
+
+! ***************************************************************************
+
+  logical function solver_norm(in_var, norm)
+    ! get_norm (variables.f90:554-576) of the device field when in_var is
+    ! the field a device-authoritative step returned: .true. and norm set.
+    ! .false. otherwise (get_norm then integrates in_var on the host).
+    use error
+    use parameters
+    implicit none
+
+    complex (pr), dimension(0:nx1,js:je,ks:ke), intent(in), target :: in_var
+    real (pr), intent(out) :: norm
+    real(c_double) :: v
+    integer(c_int) :: rc
+
+    solver_norm = .false.
+    norm = 0.0_pr
+    if (.not. dev_holds) return
+    if (.not. c_associated(held_out, c_loc(in_var))) return
+    rc = gpe_sim_norm(ctx, v)
+    if (rc /= 0) call emergency_stop('ERROR: '//last_error())
+    norm = real(v, pr)
+    solver_norm = .true.
+
+    return
+  end function solver_norm
+
+! ***************************************************************************
+
+  subroutine solver_transfer_bytes(to_device, to_host)
+    ! Field bytes moved host -> device and device -> host so far
+    implicit none
+
+    integer(c_int64_t), intent(out) :: to_device, to_host
+    integer(c_int) :: rc
+
+    to_device = 0
+    to_host = 0
+    if (.not. c_associated(ctx)) return
+    rc = gpe_transfer_bytes(ctx, to_device, to_host)
+
+    return
+  end subroutine solver_transfer_bytes
+
+! ***************************************************************************
+
   subroutine create_context()
     ! Pack the run parameters into the C struct and create the device context
     use error
@@ -111,6 +298,8 @@ module solve
 
     type(gpe_params_t) :: prm
     integer(c_int) :: rc
+    character(len=8) :: flag
+    integer :: st
 
     if (nyprocs*nzprocs /= 1) then
       call emergency_stop('ERROR: the GPU solver runs on one rank (nyprocs = nzprocs = 1).')
@@ -161,6 +350,9 @@ module solve
 
     rc = gpe_ctx_create(prm, ctx)
     if (rc /= 0) call emergency_stop('ERROR: '//last_error())
+
+    call get_environment_variable('GPE_RESIDENT', flag, status=st)
+    resident = st == 0 .and. trim(flag) == '1'
 
     return
   end subroutine create_context

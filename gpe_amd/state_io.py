@@ -39,8 +39,10 @@ def read_state(path, single=False):
     return psi, p, t, im_t, dt
 
 
-def write_run_in(path, cfg, end_time):
-    """run_params / eqn_params namelists (parameters.f90:40-56) for cfg."""
+def write_run_in(path, cfg, end_time, io=None):
+    """run_params / eqn_params namelists (parameters.f90:40-56) for cfg,
+    and io_params (parameters.f90:74-76) from the dict `io` (save_rate,
+    save_rate2, save_rate3, p_save) when given."""
     b = lambda v: ".true." if v else ".false."  # noqa: E731
     with open(path, "w") as fh:
         fh.write("&run_params\n")
@@ -53,6 +55,11 @@ def write_run_in(path, cfg, end_time):
         fh.write("  Urhs = %r\n  diss_amp = %r\n  g = %r\n  mu = %r\n" %
                  (float(cfg.urhs), float(cfg.diss_amp), float(cfg.g), float(cfg.mu)))
         fh.write("  omx = %r\n  omy = %r\n  omz = %r\n/\n" % (float(cfg.omx), float(cfg.omy), float(cfg.omz)))
+        if io:
+            fh.write("&io_params\n")
+            for k, v in io.items():
+                fh.write("  %s = %r\n" % (k, int(v) if k == "save_rate" else float(v)))
+            fh.write("/\n")
 
 
 def itos(n):

@@ -103,6 +103,28 @@ void gpe_ctx_destroy(gpe_ctx *ctx);
 int gpe_solver_step(gpe_ctx *ctx, const void *var_in, int32_t halo, void *var_out,
                     gpe_step_state *st);
 
+/* The same step with the device field authoritative across calls: the
+ * drop-in for unchanged gpe.f90 when the host does not read psi between
+ * most steps (gpe.f90:167-176 copies psi%new into psi%old and calls
+ * solver(); only its periodic I/O, 178-226, reads psi).
+ *   GPE_STEP_IN_RESIDENT   var_in's interior equals the field the context
+ *                          holds (the result of its previous step): skip
+ *                          the upload.  GPE_ESTATE if the context holds no
+ *                          field yet.
+ *   GPE_STEP_OUT_DEFERRED  do not write var_out: the context keeps the
+ *                          result and var_out keeps its old contents until
+ *                          gpe_sim_download (or a step without this flag)
+ *                          writes it.
+ * flags = 0 is gpe_solver_step. */
+#define GPE_STEP_IN_RESIDENT 1
+#define GPE_STEP_OUT_DEFERRED 2
+int gpe_solver_step_ex(gpe_ctx *ctx, const void *var_in, int32_t halo, void *var_out,
+                       gpe_step_state *st, int32_t flags);
+/* Bytes of field data this context has moved host -> device and device ->
+ * host (uploads, downloads, drop-in steps).  No reference counterpart:
+ * it shows what the device-authoritative drop-in saves. */
+int gpe_transfer_bytes(gpe_ctx *ctx, int64_t *to_device, int64_t *to_host);
+
 /* Device-resident path (no host traffic per step). */
 int gpe_sim_upload(gpe_ctx *ctx, const void *psi, int32_t host_type);
 int gpe_sim_download(gpe_ctx *ctx, void *psi, int32_t host_type);

@@ -3,37 +3,60 @@
 ! TEST HARNESS.  Built by oracle/Makefile (target dropin) against the
 ! reference's unchanged parameters.f90 and error.f90 and the drop-in
 ! gpe_amd/fortran/solve.f90.  It reads the reference's own namelists
-! run_params and eqn_params (parameters.f90:40-56) from a run.in-style file,
-! forms the grid as get_grid does (ic.f90:43-66), reads the initial state in
-! the restart format of end_state (io.f90:843-853), and runs the time loop
-! of gpe.f90:136-280 (copy psi%new into psi%old, call solver, p = p+1)
-! while t+im_t <= end_time.  The final state is written in the same format.
+! run_params, eqn_params and (if present) io_params (parameters.f90:40-76)
+! from a run.in-style file, forms the grid as get_grid does (ic.f90:43-66),
+! reads the initial state in the restart format of end_state
+! (io.f90:843-853), and runs the time loop of gpe.f90:136-280 while
+! t+im_t <= end_time: the RUNNING test, the copy of psi%new into psi%old,
+! the I/O tests in the driver's order, solver(), the SAVE test of save_run,
+! get_norm, p = p+1.  Each place where gpe.f90 would read psi for I/O
+! appends a record (p, what, t, the field it would read) to io.bin instead,
+! so two runs can be compared record by record.  get_norm is taken with
+! the drop-in's solver_norm hook where that applies (the norms go to
+! norms.txt), and every end_state record first calls solver_sync, as the
+! hooked end_state would.  The final state is written in the end_state
+! format.
+!
+! Environment (test hooks): DRIVER_STOP_AT=p deletes RUNNING at the top of
+! iteration p (a user stopping the run); DRIVER_SAVE_AT=p creates SAVE just
+! before solver() in iteration p.
 !
 ! Usage: dropin_driver <run.in> <state_in> <state_out>
 program dropin_driver
   use parameters
   use solve
+  use iso_c_binding, only : c_int64_t
   implicit none
 
   complex (pr), allocatable :: old(:,:,:), new(:,:,:)
   character(len=512) :: fin_nml, fin, fout
-  integer :: u, nx_in, ny_in, nz_in, p_in
-  real (pr) :: t_in, im_t_in
+  integer :: u, io, nx_in, ny_in, nz_in, p_in, n, m, ps, stop_at, save_at, ios
+  real (pr) :: t_in, im_t_in, norm
   complex (pr) :: dt_in
+  logical :: run_exist, save_exist, hooked
+  integer(c_int64_t) :: h2d, d2h
 
   call MPI_INIT(ierr)
   call get_command_argument(1, fin_nml)
   call get_command_argument(2, fin)
   call get_command_argument(3, fout)
 
+  ! defaults for what this harness does not read
+  save_rate = 1
+  save_rate2 = huge(1.0_pr)
+  save_rate3 = huge(1.0_pr)
+  p_save = huge(1.0_pr)
+  diagnostic = .false.
   open (newunit=u, file=trim(fin_nml), status='old')
   read (u, nml=run_params)
   read (u, nml=eqn_params)
+  read (u, nml=io_params, iostat=ios)
   close (u)
+  stop_at = env_int('DRIVER_STOP_AT')
+  save_at = env_int('DRIVER_SAVE_AT')
 
   myrank = 0
   js = 0; je = ny-1; ks = 0; ke = nz-1
-  save_rate = 1
   xl = -xr; yl = -yr; zl = -zr
   dx = (xr-xl)/nx1; dy = (yr-yl)/ny1; dz = (zr-zl)/nz1
   dx2 = dx**2; dy2 = dy**2; dz2 = dz**2
@@ -51,12 +74,67 @@ program dropin_driver
   im_t = im_t_in
   dt = dt_in
 
+  open (newunit=io, file='io.bin', access='stream', form='unformatted', status='replace')
+  open (newunit=u, file='norms.txt', status='replace')
+
+  ps = int(t/p_save)
+  n = int(t/save_rate2)
+  m = int(t/save_rate3)
   old = 0.0_pr
   do while (t+im_t <= end_time)
+    if (p == stop_at) then
+      open (99, file='RUNNING')
+      close (99, status='delete')
+    end if
+    inquire (file='RUNNING', exist=run_exist)
+    if (.not. run_exist) then
+      call solver_sync(new)
+      call record(1, new)          ! end_state(psi%new, 1)
+      exit
+    end if
     old(:,js:je,ks:ke) = new
+    if (modulo(t+im_t, abs(dt)*save_rate) < abs(dt)) then
+      call record(2, old(:,js:je,ks:ke))   ! save_energy(t, psi%old), save_linelength
+      call record(3, new)                  ! save_time(t, psi%new)
+    end if
+    if (t+im_t >= save_rate3*m) then
+      call record(4, new)          ! condensed_particles(t, psi%new)
+      m = m+1
+    end if
+    if (t+im_t >= save_rate2*n) then
+      call record(5, new)          ! save_surface / idl_surface / average
+      n = n+1
+    end if
+    if (t+im_t >= p_save*ps) then
+      call solver_sync(new)
+      call record(6, new)          ! end_state(psi%new, 0)
+      ps = ps+1
+    end if
+    if (p == save_at) then
+      open (98, file='SAVE')
+      close (98)
+    end if
     call solver(old, new)
+    inquire (file='SAVE', exist=save_exist)
+    if (save_exist) then
+      call solver_sync(new)
+      call record(7, new)          ! save_run: idl_surface(psi%new)
+      open (98, file='SAVE')
+      close (98, status='delete')
+    end if
+    hooked = solver_norm(new, norm)
+    if (hooked) write (u, '(i10,es26.17)') p, norm
+    if (modulo(t+im_t, abs(dt)*save_rate) < abs(dt)) then
+      call record(8, new)          ! get_norm(psi%new) for save_norm
+    end if
     p = p+1
   end do
+  close (u)
+  if (run_exist) then
+    call solver_sync(new)
+    call record(9, new)            ! end_state(psi%new, 1) after the loop
+  end if
+  close (io)
 
   open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   write (u) nx, ny, nz, p
@@ -64,5 +142,28 @@ program dropin_driver
   write (u) dt
   write (u) new
   close (u)
+  call solver_transfer_bytes(h2d, d2h)
+  print '(a,i0,a,i0)', 'transfer_bytes ', h2d, ' ', d2h
   call MPI_FINALIZE(ierr)
+
+contains
+
+  subroutine record(what, f)
+    integer, intent(in) :: what
+    complex (pr), intent(in) :: f(0:nx1,js:je,ks:ke)
+    write (io) p, what, t
+    write (io) f
+  end subroutine record
+
+  integer function env_int(name)
+    character(len=*), intent(in) :: name
+    character(len=32) :: v
+    integer :: st, e
+    env_int = -1
+    call get_environment_variable(name, v, status=st)
+    if (st /= 0) return
+    read (v, *, iostat=e) env_int
+    if (e /= 0) env_int = -1
+  end function env_int
+
 end program dropin_driver

@@ -90,3 +90,102 @@ def test_dropin_fp32_tracks_single_precision_oracle(tmp_path, oracle):
     want, _ = oracle.run(cfg, oracle.make_state(cfg.dt), psi, n)
     assert p == n
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5
+
+
+def _records(path, cfg, single=False):
+    """io.bin of the driver: (p, what, t, field) per place gpe.f90 reads psi."""
+    raw = open(path, "rb").read()
+    rs = 4 if single else 8
+    fb = cfg.nx * cfg.ny * cfg.nz * 2 * rs
+    out, o = [], 0
+    while o < len(raw):
+        p, what = np.frombuffer(raw, np.int32, 2, o)
+        t = float(np.frombuffer(raw, np.float32 if single else np.float64, 1, o + 8)[0])
+        o += 8 + rs
+        f = np.frombuffer(raw, np.complex64 if single else np.complex128, cfg.nx * cfg.ny * cfg.nz, o)
+        o += fb
+        out.append((int(p), int(what), t, f.reshape(cfg.shape)))
+    return out
+
+
+def _run_mode(tmp_path, exe, cfg, psi, nsteps, io, resident, **env_extra):
+    d = tmp_path / ("resident" if resident else "plain")
+    d.mkdir()
+    write_run_in(str(d / "run.in"), cfg, (nsteps - 0.5) * cfg.tau, io=io)
+    write_state(str(d / "end_state0000.dat"), psi, dt=cfg.dt)
+    open(d / "RUNNING", "w").close()
+    env = dict(os.environ)
+    env.pop("GPE_RESIDENT", None)
+    if resident:
+        env["GPE_RESIDENT"] = "1"
+    env.update({k: str(v) for k, v in env_extra.items()})
+    r = subprocess.run([exe, "run.in", "end_state0000.dat", "out.dat"], cwd=str(d), capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    h2d, d2h = [int(v) for v in r.stdout.split("transfer_bytes")[1].split()[:2]]
+    return d, h2d, d2h
+
+
+IO = {"save_rate": 5, "save_rate2": 0.07, "save_rate3": 0.11, "p_save": 0.13}
+
+
+@pytest.mark.gpu
+@need_bins
+@pytest.mark.parametrize("scheme", ["rk4", "rk45"])
+def test_resident_dropin_equals_plain(tmp_path, oracle, scheme):
+    """GPE_RESIDENT=1: the field stays on the GPU; every place the driver
+    reads psi (the I/O schedule of gpe.f90) sees the same field as the plain
+    drop-in, and the final state is the same, bit for bit, with one upload
+    and far fewer downloads."""
+    cfg = cfg_for(scheme=scheme, bcs=1)
+    psi = ic.vortex_ring(cfg, 1.5) * (1 + 0.02 * np.random.default_rng(5).standard_normal(cfg.shape))
+    n = 40
+    print("[case] resident drop-in", scheme, flush=True)
+    dp, h2d_p, d2h_p = _run_mode(tmp_path, BIN_D, cfg, psi, n, IO, False)
+    dr, h2d_r, d2h_r = _run_mode(tmp_path, BIN_D, cfg, psi, n, IO, True)
+    assert (dp / "io.bin").read_bytes() == (dr / "io.bin").read_bytes()
+    assert (dp / "out.dat").read_bytes() == (dr / "out.dat").read_bytes()
+    recs = _records(str(dr / "io.bin"), cfg)
+    assert {w for _, w, _, _ in recs} >= {2, 3, 4, 5, 6, 8, 9}
+    field = cfg.nx * cfg.ny * cfg.nz * 16
+    steps = read_state(str(dr / "out.dat"))[1]
+    assert h2d_p == d2h_p == steps * field
+    assert h2d_r == field
+    assert 0 < d2h_r < d2h_p // 2
+    # get_norm through the hook: the device field's norm, at every step
+    norms = {int(a): float(b) for a, b in (ln.split() for ln in (dr / "norms.txt").read_text().splitlines())}
+    assert len(norms) == steps
+    for p, w, _, f in recs:
+        if w == 8:  # the field get_norm(psi%new) was given, fresh at this step
+            assert norms[p] == pytest.approx(oracle.norm(cfg, np.ascontiguousarray(f)), rel=1e-12)
+
+
+@pytest.mark.gpu
+@need_bins
+def test_resident_dropin_stop_and_save_files(tmp_path):
+    """A run stopped by deleting RUNNING and a SAVE request mid-run: with
+    the solver_sync hook in end_state and save_run, both read the current
+    field."""
+    cfg = cfg_for(scheme="rk4", bcs=2)
+    psi = ic.vortex_ring(cfg, 1.5)
+    print("[case] resident drop-in stop / save", flush=True)
+    hooks = {"DRIVER_STOP_AT": 23, "DRIVER_SAVE_AT": 9}
+    dp, _, _ = _run_mode(tmp_path, BIN_D, cfg, psi, 40, IO, False, **hooks)
+    dr, _, d2h = _run_mode(tmp_path, BIN_D, cfg, psi, 40, IO, True, **hooks)
+    assert (dp / "io.bin").read_bytes() == (dr / "io.bin").read_bytes()
+    assert (dp / "out.dat").read_bytes() == (dr / "out.dat").read_bytes()
+    recs = _records(str(dr / "io.bin"), cfg)
+    assert [w for _, w, _, _ in recs].count(7) == 1 and recs[-1][1] == 1 and recs[-1][0] == 23
+
+
+@pytest.mark.gpu
+@need_bins
+def test_resident_dropin_fp32(tmp_path):
+    cfg = cfg_for(scheme="rk4", precision="fp32")
+    psi = ic.vortex_ring(cfg, 1.5).astype(np.complex64)
+    print("[case] resident drop-in fp32", flush=True)
+    dp, _, _ = _run_mode(tmp_path, BIN_S, cfg, psi, 20, IO, False)
+    dr, h2d, _ = _run_mode(tmp_path, BIN_S, cfg, psi, 20, IO, True)
+    assert (dp / "io.bin").read_bytes() == (dr / "io.bin").read_bytes()
+    assert (dp / "out.dat").read_bytes() == (dr / "out.dat").read_bytes()
+    assert h2d == cfg.nx * cfg.ny * cfg.nz * 8
