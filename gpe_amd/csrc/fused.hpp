@@ -522,7 +522,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
                 stc<T>(B.psi_out, s, nf);
                 np = todbl(nf);
               }
-              const float e = sqrtf(err.re * err.re + err.im * err.im) / sqrtf(sc.re * sc.re + sc.im * sc.im);
+              const float e = cabs_(cdiv(err, sc));  // abs(err/scale) (solve.f90:184)
               acc_emax = ((double)e > acc_emax || e != e) ? (double)e : acc_emax;
             }
             const double wn = nweight(gi, nx, B.w1, B.w2, B.w3) * nweight(gj[p], ny, B.w1, B.w2, B.w3) *

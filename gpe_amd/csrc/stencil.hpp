@@ -113,8 +113,26 @@ __device__ __forceinline__ cx<double> cdiv(cx<double> a, cx<double> b) {
   return mk(ldexp((ac + bd) / den, -e), ldexp((bc - ad) / den, -e));
 }
 
+// the same in single precision (__divsc3): the scaling keeps |b|^2 from
+// underflowing when the rk45 error scale is as small as its 1e-30 floor
+__device__ __forceinline__ cx<float> cdiv(cx<float> a, cx<float> b) {
+  float c = b.re, d = b.im;
+  float m = fmaxf(fabsf(c), fabsf(d));
+  int e = 0;
+  if (m != 0.0f && isfinite(m)) {
+    e = ilogbf(m);
+    c = ldexpf(c, -e);
+    d = ldexpf(d, -e);
+  }
+  float cc = c * c, dd = d * d;
+  float den = cc + dd;
+  float ac = a.re * c, bd = a.im * d, bc = a.im * c, ad = a.re * d;
+  return mk(ldexpf((ac + bd) / den, -e), ldexpf((bc - ad) / den, -e));
+}
+
+// abs() of a complex: hypot, as flang emits it (no underflow of |a|^2)
 __device__ __forceinline__ double cabs_(cx<double> a) { return hypot(a.re, a.im); }
-__device__ __forceinline__ float cabs_(cx<float> a) { return sqrtf(a.re * a.re + a.im * a.im); }
+__device__ __forceinline__ float cabs_(cx<float> a) { return hypotf(a.re, a.im); }
 
 // Index map for a ghost index i in [-R, n-1+R].
 __device__ __forceinline__ int bcmap(int i, int n, int periodic) {
@@ -617,7 +635,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
                   stc<T>(A.psi_out, s, nf);
                   np = todbl(nf);
                 }
-                const float e = sqrtf(err.re * err.re + err.im * err.im) / sqrtf(sc.re * sc.re + sc.im * sc.im);
+                // abs(err/scale) (solve.f90:184) in single precision
+                const float e = cabs_(cdiv(err, sc));
                 acc_emax = ((double)e > acc_emax || e != e) ? (double)e : acc_emax;
               }
               const double wn = nweight(gi, nx, A.w1, A.w2, A.w3) * nweight(gj[p], ny, A.w1, A.w2, A.w3) *
