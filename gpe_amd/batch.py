@@ -85,6 +85,25 @@ class Batch:
         self.targets[i.value] = int(nsteps)
         return i.value
 
+    def add_random_phase(self, cfg: RunConfig, seed, nsteps, kmax=0.5, dt=None, t=0.0, im_t=0.0):
+        """Add an instance whose initial field is the random-phase state,
+        built on the device in the instance's memory; returns its id, or
+        None when it does not fit the HBM budget."""
+        if cfg.precision not in ("fp64", "fp32c"):
+            raise ValueError("batches run fp64 or fp32c instances")
+        st = _lib.GpeStepState()
+        d = complex(cfg.dt if dt is None else dt)
+        st.dt_re, st.dt_im, st.t, st.im_t = d.real, d.imag, t, im_t
+        i = ctypes.c_int32()
+        rc = self._L.gpe_batch_add_random_phase(self._b, ctypes.byref(make_params(cfg)), ctypes.c_uint64(int(seed)),
+                                                float(kmax), ctypes.byref(st), int(nsteps), ctypes.byref(i))
+        if rc == _lib.GPE_EFULL:
+            return None
+        _lib.check(rc)
+        self.cfgs[i.value] = cfg
+        self.targets[i.value] = int(nsteps)
+        return i.value
+
     def remove(self, iid):
         _lib.check(self._L.gpe_batch_remove(self._b, int(iid)))
         self.cfgs.pop(iid, None)

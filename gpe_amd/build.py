@@ -27,7 +27,8 @@ N_VARIANTS = 5  # kTiles in csrc/launch.h
 UNITS = [("gpe_misc.hip", "gpe_misc", ["-ffp-contract=off"]),
          ("gpe_capi.cpp", "gpe_capi", ["-ffp-contract=off", "-x", "hip"]),
          ("gpe_batch.cpp", "gpe_batch", ["-ffp-contract=off", "-x", "hip"]),
-         ("host_common.cpp", "host_common", ["-ffp-contract=off", "-x", "hip"])]
+         ("host_common.cpp", "host_common", ["-ffp-contract=off", "-x", "hip"]),
+         ("gpe_ic.cpp", "gpe_ic", ["-ffp-contract=off", "-x", "hip"])]
 for _v in range(N_VARIANTS):
     # stage outputs are written with nontemporal stores (-DGPE_NT_STORE): they
     # are not re-read within a sweep, and keeping them out of L2 leaves room
@@ -68,7 +69,10 @@ def build(verbose=True, extra_flags=(), lib_path=None, obj_dir=None):
         objs = list(ex.map(_compile, UNITS))
     out = lib_path or LIB
     tmp = out + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    # hipFFT (the random-phase initial state) from the ROCm install
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "lib")
+    cmd = ([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
+           ["-L" + rocm_lib, "-lhipfft", "-Wl,-rpath," + rocm_lib])
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))

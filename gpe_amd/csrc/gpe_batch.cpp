@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/gpe_hip.h"
+#include "gpe_ic.h"
 #include "host_common.h"
 #include "launch.h"
 
@@ -608,9 +609,43 @@ int gpe_batch_footprint(const gpe_params *params, int64_t *bytes) {
   return GPE_OK;
 }
 
+namespace {
+// The initial field of a new instance: a host array, or the random-phase
+// state built on the device.
+struct FieldSource {
+  const void *psi = nullptr;
+  int32_t host_type = 0;
+  bool random_phase = false;
+  uint64_t seed = 0;
+  double kmax = 0.0;
+};
+
+int add_instance(gpe_batch *b, const gpe_params *params, const FieldSource &src, const gpe_step_state *st,
+                 int64_t nsteps, int32_t *id);
+}  // namespace
+
 int gpe_batch_add(gpe_batch *b, const gpe_params *params, const void *psi, int32_t host_type,
                   const gpe_step_state *st, int64_t nsteps, int32_t *id) {
   if (!b || !psi || !st || !id) return fail(GPE_EINVAL, "NULL argument");
+  FieldSource src;
+  src.psi = psi;
+  src.host_type = host_type;
+  return add_instance(b, params, src, st, nsteps, id);
+}
+
+int gpe_batch_add_random_phase(gpe_batch *b, const gpe_params *params, uint64_t seed, double kmax,
+                               const gpe_step_state *st, int64_t nsteps, int32_t *id) {
+  if (!b || !st || !id) return fail(GPE_EINVAL, "NULL argument");
+  FieldSource src;
+  src.random_phase = true;
+  src.seed = seed;
+  src.kmax = kmax;
+  return add_instance(b, params, src, st, nsteps, id);
+}
+
+namespace {
+int add_instance(gpe_batch *b, const gpe_params *params, const FieldSource &src, const gpe_step_state *st,
+                 int64_t nsteps, int32_t *id) {
   int rc = check_batch_params(params);
   if (rc) return rc;
   if (nsteps < 1) return fail(GPE_EINVAL, "nsteps must be >= 1");
@@ -653,8 +688,12 @@ int gpe_batch_add(gpe_batch *b, const gpe_params *params, const void *psi, int32
       for (int m = 0; m < 5; ++m) I.k[m] = p + m * fb;
       if (params->precision != GPE_FP32C) I.scale = p + 5 * fb;
     }
-    rc = field_upload(params->precision, (size_t)params->nx * params->ny, params->nz, psi, host_type,
-                      (size_t)params->nx * params->ny, 0, I.buf[0], I.lob[0]);
+    if (src.random_phase)
+      rc = random_phase_device(params->nx, params->ny, params->nz, params->dx, params->dy, params->dz, src.kmax,
+                               src.seed, params->precision, I.buf[0], I.lob[0], b->stream);
+    else
+      rc = field_upload(params->precision, (size_t)params->nx * params->ny, params->nz, src.psi, src.host_type,
+                        (size_t)params->nx * params->ny, 0, I.buf[0], I.lob[0]);
     if (rc) {
       give_block(b, I.block, bytes);
       return rc;
@@ -707,6 +746,7 @@ int gpe_batch_add(gpe_batch *b, const gpe_params *params, const void *psi, int32
   *id = I.id;
   return GPE_OK;
 }
+}  // namespace
 
 int gpe_batch_remove(gpe_batch *b, int32_t id) {
   if (!b) return fail(GPE_EINVAL, "batch is NULL");

@@ -17,6 +17,7 @@
 
 #include "../../include/gpe_hip.h"
 #include "host_common.h"
+#include "gpe_ic.h"
 #include "launch.h"
 
 using namespace gpe;
@@ -800,6 +801,17 @@ int gpe_sim_linelength(gpe_ctx *c, double *length) {
   for (double v : part) sum += v;
   sum += c->prec == GPE_FP32 ? ll_singles<float>(w, sg) : ll_singles<double>(w, sg);
   *length = c->prec == GPE_FP32 ? (double)((float)sum * (float)c->P.dx) : sum * c->P.dx;
+  return GPE_OK;
+}
+
+int gpe_sim_random_phase(gpe_ctx *c, uint64_t seed, double kmax) {
+  if (!c) return fail(GPE_EINVAL, "NULL context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc = random_phase_device(c->P.nx, c->P.ny, c->P.nz, c->P.dx, c->P.dy, c->P.dz, kmax, seed, c->prec, c->psi,
+                               c->comp ? c->lo : nullptr, c->stream);
+  if (rc) return rc;
+  c->uploaded = true;
+  c->hist_len = 0;
   return GPE_OK;
 }
 

@@ -45,10 +45,13 @@ class _Ready:
 
 class Server:
     def __init__(self, hbm_budget=0, max_sites=None, target_chunk_s=0.05, ic_workers=4, prefetch=8,
-                 deliver=None):
+                 deliver=None, device_ic=True):
         """hbm_budget: device bytes for resident simulations (0: 90% of free).
         max_sites: cap on resident grid points (None: no cap).
-        deliver(request, psi, info): called with each finished result."""
+        deliver(request, psi, info): called with each finished result.
+        device_ic: build random-phase initial states on the device (hipFFT)
+        instead of on the host (the other states are built on the host by
+        ic_workers threads, `prefetch` requests ahead)."""
         self.batch = Batch(hbm_budget)
         self.budget = self.batch.stats()["budget"]
         self.max_sites = max_sites
@@ -56,6 +59,7 @@ class Server:
         self.pool = ThreadPoolExecutor(max_workers=ic_workers)
         self.prefetch = prefetch
         self.deliver = deliver
+        self.device_ic = device_ic
 
     def close(self):
         self.pool.shutdown(wait=True)
@@ -101,9 +105,12 @@ class Server:
         su_done = 0.0
         last_log = 0.0
 
+        def on_device(r):
+            return self.device_ic and r.ic_kind == "random_phase"
+
         def prefetch():
             for r in list(queue)[: self.prefetch]:
-                if r.rid not in futures:
+                if r.rid not in futures and not on_device(r):
                     futures[r.rid] = self.pool.submit(r.initial_state)
 
         while True:
@@ -119,12 +126,18 @@ class Server:
                     break
                 if _batch.footprint(r.cfg) > self.budget:
                     raise ValueError("request %d can never fit the HBM budget" % r.rid)
-                f = futures.pop(r.rid, None)
-                psi = f.result() if f is not None else r.initial_state()
-                iid = self.batch.add(r.cfg, psi, r.steps)
-                if iid is None:  # does not fit now: wait for memory
-                    futures[r.rid] = _Ready(psi)
-                    break
+                if on_device(r):
+                    # the random-phase state is built in the instance's memory
+                    iid = r.add_to(self.batch, device_ic=True)
+                    if iid is None:  # does not fit now: wait for memory
+                        break
+                else:
+                    f = futures.pop(r.rid, None)
+                    psi = f.result() if f is not None else r.initial_state()
+                    iid = self.batch.add(r.cfg, psi, r.steps)
+                    if iid is None:  # does not fit now: wait for memory
+                        futures[r.rid] = _Ready(psi)
+                        break
                 queue.popleft()
                 running[iid] = (r, clock(), sites)
                 resident_sites += sites
@@ -218,9 +231,10 @@ def main():
     ap.add_argument("--budget-gib", type=float, default=0.0)
     ap.add_argument("--max-msites", type=float, default=64.0, help="resident grid points cap, millions")
     ap.add_argument("--no-drain", action="store_true")
+    ap.add_argument("--host-ic", action="store_true", help="build random-phase states on the host")
     a = ap.parse_args()
     reqs = request_stream(a.rate, a.duration, a.seed)
-    with Server(hbm_budget=int(a.budget_gib * 2 ** 30), max_sites=a.max_msites * 1e6) as s:
+    with Server(hbm_budget=int(a.budget_gib * 2 ** 30), max_sites=a.max_msites * 1e6, device_ic=not a.host_ic) as s:
         m = s.serve(reqs, duration=a.duration, drain=not a.no_drain,
                     log=lambda *x: print("[serve]", *x, file=sys.stderr, flush=True))
     m.update({"rate": a.rate, "duration": a.duration, "requests": len(reqs)})

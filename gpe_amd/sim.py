@@ -160,6 +160,12 @@ class Simulation:
         """Total vortex-line length (linelength, variables.f90:896-1162)."""
         return self._scalar(self._L.gpe_sim_linelength)
 
+    def random_phase(self, seed, kmax=0.5):
+        """Replace the resident field by the random-phase state built on the
+        device (the state ic.random_phase builds on the host: unit modes
+        with |k| <= kmax and SplitMix64 phases, inverse FFT, <|psi|^2> = 1)."""
+        _lib.check(self._L.gpe_sim_random_phase(self._ctx, ctypes.c_uint64(int(seed)), float(kmax)))
+
     def zeros(self, which="zeros"):
         """Vortex-core points of the resident field as the reference's
         zeros files hold them (io.f90:886-1393): which = "zeros"
@@ -236,3 +242,18 @@ class Simulation:
     def autotune(self, reps=3):
         _lib.check(self._L.gpe_autotune(self._ctx, int(reps)))
         return self.tune_get()
+
+
+def random_phase_modes(cfg: RunConfig, seed, kmax=0.5):
+    """Flat indices and unit phasors of the random-phase state's modes, as
+    the device generator forms them (host only, no GPU work)."""
+    L = _lib.lib()
+    args = (int(cfg.nx), int(cfg.ny), int(cfg.nz), float(cfg.dx), float(cfg.dy), float(cfg.dz), float(kmax),
+            ctypes.c_uint64(int(seed)))
+    n = ctypes.c_int64()
+    _lib.check(L.gpe_random_phase_modes(*args, None, None, 0, ctypes.byref(n)))
+    idx = np.zeros(max(n.value, 1), dtype=np.int64)
+    ph = np.zeros(max(n.value, 1), dtype=np.complex128)
+    _lib.check(L.gpe_random_phase_modes(*args, idx.ctypes.data_as(ctypes.c_void_p), ph.ctypes.data_as(ctypes.c_void_p),
+                                        n.value, ctypes.byref(n)))
+    return idx[:n.value], ph[:n.value]

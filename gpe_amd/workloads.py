@@ -39,6 +39,15 @@ class Instance:
     radius: float
     seed: int
 
+    def add_to(self, batch, device_ic=True):
+        """Admit this request into a Batch: the random-phase state is built
+        on the device (no host FFT, no upload) unless device_ic is False;
+        the others are built on the host and uploaded.  Returns the
+        instance id, or None when it does not fit the HBM budget."""
+        if device_ic and self.ic_kind == "random_phase":
+            return batch.add_random_phase(self.cfg, self.seed, self.steps, kmax=0.5)
+        return batch.add(self.cfg, self.initial_state(), self.steps)
+
     def initial_state(self):
         return ic.trapped_ring(self.cfg, self.radius, mu_ic=5.0)
 
@@ -65,6 +74,15 @@ class Request:
     steps: int
     ic_kind: str        # ring | line_pair | random_phase
     seed: int
+
+    def add_to(self, batch, device_ic=True):
+        """Admit this request into a Batch: the random-phase state is built
+        on the device (no host FFT, no upload) unless device_ic is False;
+        the others are built on the host and uploaded.  Returns the
+        instance id, or None when it does not fit the HBM budget."""
+        if device_ic and self.ic_kind == "random_phase":
+            return batch.add_random_phase(self.cfg, self.seed, self.steps, kmax=0.5)
+        return batch.add(self.cfg, self.initial_state(), self.steps)
 
     def initial_state(self):
         c = self.cfg

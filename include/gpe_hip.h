@@ -161,6 +161,21 @@ typedef struct gpe_zero_point {
   double x, y, z;    /* the point */
 } gpe_zero_point;
 int gpe_sim_zeros(gpe_ctx *ctx, int32_t which, gpe_zero_point *out, int64_t cap, int64_t *count);
+/* Random-phase initial state built on the device, replacing the resident
+ * field.  The reference's random_phase (ic.f90:195-274) gives the modes
+ * inside a sphere of wavenumbers a fixed amplitude and a random phase and
+ * transforms back with 2DECOMP&FFT (ic.f90:910-939), phases drawn from the
+ * Fortran generator seeded by the clock.  Here: every mode
+ * k = 2 pi m/(n d) (m in fftfreq order) with |k| <= kmax gets amplitude 1
+ * and phase 2 pi u, u from SplitMix64(seed) in the order of the modes' flat
+ * index; inverse FFT (hipFFT); rescaled so <|psi|^2> = 1; converted to the
+ * context's storage form. */
+int gpe_sim_random_phase(gpe_ctx *ctx, uint64_t seed, double kmax);
+/* The mode list of that state, computed on the host (no device work):
+ * writes min(count, cap) flat indices (i + nx*(j + ny*k)) and unit phasors
+ * (re, im pairs) and sets *count. */
+int gpe_random_phase_modes(int32_t nx, int32_t ny, int32_t nz, double dx, double dy, double dz, double kmax,
+                           uint64_t seed, int64_t *flat_index, double *phasor, int64_t cap, int64_t *count);
 /* renormalise (variables.f90:715-726): psi = sqrt(nn)*psi/sqrt(norm). */
 int gpe_sim_renormalise(gpe_ctx *ctx, double nn, double norm);
 /* f(psi) of the resident field into a host buffer (get_rhs, solve.f90:382). */
@@ -232,6 +247,10 @@ int gpe_batch_footprint(const gpe_params *params, int64_t *bytes);
  * GPE_EFULL, adding nothing, when it does not fit the budget. */
 int gpe_batch_add(gpe_batch *b, const gpe_params *params, const void *psi, int32_t host_type,
                   const gpe_step_state *st, int64_t nsteps, int32_t *id);
+/* gpe_batch_add with the random-phase state of gpe_sim_random_phase built
+ * on the device in the instance's memory (no host field, no upload). */
+int gpe_batch_add_random_phase(gpe_batch *b, const gpe_params *params, uint64_t seed, double kmax,
+                               const gpe_step_state *st, int64_t nsteps, int32_t *id);
 int gpe_batch_remove(gpe_batch *b, int32_t id);
 /* Enqueue `iterations` iterations (asynchronous).  0 only prepares: it
  * rebuilds the work tables and captures the iteration graph if the set of

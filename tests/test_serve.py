@@ -88,9 +88,8 @@ def test_served_results_equal_alone(budget_fields):
         m = s.serve(reqs, clock=FakeClock())
     assert m["completed"] == len(reqs) and m["ok"] == len(reqs), m
     for r in reqs:
-        psi0 = r.initial_state()
         with Batch() as b:
-            i = b.add(r.cfg, psi0, r.steps)
+            i = r.add_to(b)  # the same initial-state path as the server's
             b.run_to_completion(chunk=5)
             want = b.download(i)
             q = b.query(i)
@@ -123,7 +122,7 @@ def test_checkpoint_and_resume_equal_alone(tmp_path):
     assert sorted(got) == sorted(r.rid for r in arrived), (sorted(got), m1, m2)
     for r in arrived:
         with Batch() as b:
-            i = b.add(r.cfg, r.initial_state(), r.steps)
+            i = r.add_to(b)
             b.run_to_completion(chunk=5)
             want = b.download(i)
         assert np.array_equal(got[r.rid], want), r
