@@ -54,3 +54,24 @@ def test_invalid_params_rejected_without_gpu_work():
     p.scheme = 3
     assert L.gpe_ctx_create(ctypes.byref(p), ctypes.byref(ctx)) == -1
     assert b"Unrecognised time stepping scheme" in L.gpe_last_error()
+
+
+def test_new_entry_points_reject_bad_arguments_without_gpu_work():
+    L = _lib.lib()
+    n = ctypes.c_int64()
+    i64 = ctypes.c_int64
+    # NULL context / outputs
+    assert L.gpe_sim_zeros(None, 0, None, 0, ctypes.byref(n)) == -1
+    assert L.gpe_sim_random_phase(None, ctypes.c_uint64(1), 0.5) == -1
+    assert L.gpe_transfer_bytes(None, ctypes.byref(i64()), ctypes.byref(i64())) == -1
+    st = _lib.GpeStepState()
+    assert L.gpe_solver_step_ex(None, None, 2, None, ctypes.byref(st), 3) == -1
+    # the mode list is host-only: bad grids and kmax are refused, a good one counts
+    for args in [(0, 8, 8, 0.5, 0.5, 0.5, 0.5), (8, 8, 8, -0.5, 0.5, 0.5, 0.5), (8, 8, 8, 0.5, 0.5, 0.5, -1.0)]:
+        assert L.gpe_random_phase_modes(*args, ctypes.c_uint64(1), None, None, 0, ctypes.byref(n)) == -1
+    assert b"random phase" in L.gpe_last_error()
+    assert L.gpe_random_phase_modes(8, 8, 8, 0.5, 0.5, 0.5, 0.0, ctypes.c_uint64(1), None, None, 0,
+                                    ctypes.byref(n)) == 0
+    assert n.value == 1  # kmax = 0: the zero mode only
+    assert L.gpe_random_phase_modes(8, 8, 8, 0.5, 0.5, 0.5, 0.5, ctypes.c_uint64(1), None, None, 4,
+                                    ctypes.byref(n)) == -1  # cap without buffers
