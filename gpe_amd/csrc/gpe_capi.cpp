@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -801,6 +802,43 @@ int gpe_sim_linelength(gpe_ctx *c, double *length) {
   for (double v : part) sum += v;
   sum += c->prec == GPE_FP32 ? ll_singles<float>(w, sg) : ll_singles<double>(w, sg);
   *length = c->prec == GPE_FP32 ? (double)((float)sum * (float)c->P.dx) : sum * c->P.dx;
+  return GPE_OK;
+}
+
+int gpe_sim_imprint_vortex_line(gpe_ctx *c, const gpe_line_param *vl, double scal) {
+  if (!c || !vl) return fail(GPE_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  if (vl->dir < 0 || vl->dir > 2) return fail(GPE_EINVAL, "line direction must be 0 (x), 1 (y) or 2 (z)");
+  LineArgs L{};
+  // epsilon(ll) of the reference's real kind (ic.f90:840-853)
+  const double eps = c->prec == GPE_FP32 ? (double)FLT_EPSILON : DBL_EPSILON;
+  if (std::fabs(vl->ll) < eps) {
+    if (std::fabs(vl->amp1) >= eps || std::fabs(vl->amp2) >= eps)
+      return fail(GPE_EINVAL, "vortex line disturbance wavelength is zero, but amplitude is non-zero.");
+    L.safe1 = L.safe2 = 1.0;
+  }
+  L.xl = c->P.xl;
+  L.yl = c->P.yl;
+  L.zl = c->P.zl;
+  L.dx = c->P.dx;
+  L.dy = c->P.dy;
+  L.dz = c->P.dz;
+  L.x0 = vl->x0;
+  L.y0 = vl->y0;
+  L.z0 = vl->z0;
+  L.a1 = vl->amp1;
+  L.a2 = vl->amp2;
+  L.ll = vl->ll;
+  L.sgn = vl->sgn;
+  L.scal = scal;
+  L.nx = c->P.nx;
+  L.ny = c->P.ny;
+  L.nz = c->P.nz;
+  L.dir = vl->dir;
+  L.phase_only = vl->imprint_phase != 0;
+  hipError_t e = launch_imprint(c->prec, c->psi, c->comp ? c->lo : nullptr, L, c->stream);
+  if (e != hipSuccess) return fail(GPE_EHIP, "imprint launch: %s", hipGetErrorString(e));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return GPE_OK;
 }
 

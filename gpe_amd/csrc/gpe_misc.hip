@@ -154,6 +154,16 @@ hipError_t launch_renorm_f32(void *psi, void *lo, size_t n, double a, double b, 
   return hipGetLastError();
 }
 
+hipError_t launch_imprint(int prec, void *psi, void *lo, const LineArgs &L, hipStream_t s) {
+  if (prec == 0)  // GPE_FP64
+    hipLaunchKernelGGL((imprint_kernel<double, 0>), dim3(PW_BLOCKS), dim3(256), 0, s, psi, nullptr, L);
+  else if (prec == 1)  // GPE_FP32: the reference built in single precision
+    hipLaunchKernelGGL((imprint_kernel<float, 1>), dim3(PW_BLOCKS), dim3(256), 0, s, psi, nullptr, L);
+  else
+    hipLaunchKernelGGL((imprint_kernel<double, 2>), dim3(PW_BLOCKS), dim3(256), 0, s, psi, lo, L);
+  return hipGetLastError();
+}
+
 // HBM bandwidth probes: read, write and copy of float4 arrays, grid-stride.
 static __global__ void __launch_bounds__(256) bw_read(const float4 *__restrict__ a, size_t n, float *sink) {
   float acc = 0.f;

@@ -91,6 +91,10 @@ hipError_t launch_zeros(int prec, const void *psi, const void *lo, int nx, int n
                         const double *dxyz, int which, ZeroPoint *out, int *count, int cap, int nblocks,
                         hipStream_t s);
 
+// imprint_vortex_line over the whole field (pointwise.hpp); prec as
+// GPE_FP64 / GPE_FP32 / GPE_FP32C (lo = low word of the compensated form).
+hipError_t launch_imprint(int prec, void *psi, void *lo, const LineArgs &L, hipStream_t s);
+
 // HBM read / write / copy bandwidth (GB/s) of float4 arrays of `bytes`.
 hipError_t probe_bandwidth(size_t bytes, int reps, double *gbs);
 

@@ -64,4 +64,63 @@ __global__ void __launch_bounds__(256) renorm_kernel_f32(void *psi, void *lo, si
   }
 }
 
+// imprint_vortex_line (variables.f90:1166-1180): psi = abs(psi) *
+// vortex_line(vl), vortex_line = exp(i theta) (times amp(r) unless the
+// line imprints its phase only, ic.f90:332-353), theta and r of get_theta /
+// get_r (ic.f90:735-889) for a line along `dir` displaced by
+// (a1 sin, a2 cos)(2 pi (c - c0)/(ll + safe)).  T is the arithmetic type:
+// double for FP64 and the compensated form (on hi + lo), float for FP32.
+template <typename T, int PREC>
+__global__ void __launch_bounds__(256) imprint_kernel(void *psi, void *lo, LineArgs L) {
+  const size_t n = (size_t)L.nx * L.ny * L.nz;
+  const T twopi = (T)2.0 * (T)3.14159265358979323846;  // 2.0_pr*pi (parameters.f90:124)
+  const T x0 = (T)L.x0, y0 = (T)L.y0, z0 = (T)L.z0, a1 = (T)L.a1, a2 = (T)L.a2, scal = (T)L.scal;
+  const T w1 = (T)L.ll + (T)L.safe1, w2 = (T)L.ll + (T)L.safe2, sgn = (T)L.sgn;
+  for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(s % L.nx);
+    const size_t rr = s / L.nx;
+    const int j = (int)(rr % L.ny);
+    const int k = (int)(rr / L.ny);
+    const T x = (T)L.xl + (T)i * (T)L.dx;
+    const T y = (T)L.yl + (T)j * (T)L.dy;
+    const T z = (T)L.zl + (T)k * (T)L.dz;
+    T A, B;
+    if (L.dir == 0) {  // 'x': atan2(z.., y..)
+      const T t = x - x0;
+      A = y - y0 - a1 * sin(twopi * t / w1);
+      B = z - z0 - a2 * cos(twopi * t / w2);
+    } else if (L.dir == 1) {  // 'y': atan2(x.., z..)
+      const T t = y - y0;
+      A = z - z0 - a1 * sin(twopi * t / w1);
+      B = x - x0 - a2 * cos(twopi * t / w2);
+    } else {  // 'z': atan2(y.., x..)
+      const T t = z - z0;
+      A = x - x0 - a1 * sin(twopi * t / w1);
+      B = y - y0 - a2 * cos(twopi * t / w2);
+    }
+    const T theta = sgn * atan2(scal * B, scal * A);
+    T vr = cos(theta), vi = sin(theta);
+    if (!L.phase_only) {
+      const T r = sqrt(scal * (A * A) + scal * (B * B));
+      const T am = (T)1.0 - exp((T)-0.7 * pow(r, (T)1.15));  // amp (ic.f90:314-328)
+      vr = am * vr;
+      vi = am * vi;
+    }
+    if constexpr (PREC == 0) {
+      const double m = cabs_(ldc<double>(psi, s));
+      stc<double>(psi, s, mk(m * vr, m * vi));
+    } else if constexpr (PREC == 1) {
+      const float m = cabs_(ldc<float>(psi, s));
+      stc<float>(psi, s, mk(m * vr, m * vi));
+    } else {
+      const cx<double> v = todbl(ldc<float>(psi, s)) + todbl(ldc<float>(lo, s));
+      const double m = hypot(v.re, v.im);
+      const cx<double> w = mk(m * vr, m * vi);
+      const cx<float> hi = mk((float)w.re, (float)w.im);
+      stc<float>(psi, s, hi);
+      stc<float>(lo, s, mk((float)(w.re - (double)hi.re), (float)(w.im - (double)hi.im)));
+    }
+  }
+}
+
 }  // namespace gpe

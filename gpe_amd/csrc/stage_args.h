@@ -115,6 +115,14 @@ struct LinePoint {
   double x, y, z;
 };
 
+// A vortex line to imprint (type line_param, parameters.in:33-43) on a
+// grid, with the reference's safe wavelengths already applied.
+struct LineArgs {
+  double xl, yl, zl, dx, dy, dz;
+  double x0, y0, z0, a1, a2, ll, sgn, scal, safe1, safe2;
+  int nx, ny, nz, dir, phase_only;
+};
+
 // A vortex-core point (zeros pass, see diag.hpp); the layout of
 // gpe_zero_point in include/gpe_hip.h.
 struct ZeroPoint {

@@ -160,6 +160,16 @@ class Simulation:
         """Total vortex-line length (linelength, variables.f90:896-1162)."""
         return self._scalar(self._L.gpe_sim_linelength)
 
+    def imprint_vortex_line(self, line, scal=1.0):
+        """imprint_vortex_line (variables.f90:1166-1180) on the resident
+        field; line is a dict with x0, y0, z0, amp1, amp2, ll, sgn,
+        dir ('x'|'y'|'z') and imprint_phase (type line_param)."""
+        vl = _lib.GpeLineParam(float(line.get("x0", 0.0)), float(line.get("y0", 0.0)), float(line.get("z0", 0.0)),
+                               float(line.get("amp1", 0.0)), float(line.get("amp2", 0.0)),
+                               float(line.get("ll", 0.0)), float(line.get("sgn", 1.0)),
+                               "xyz".index(line.get("dir", "z")), int(bool(line.get("imprint_phase", True))))
+        _lib.check(self._L.gpe_sim_imprint_vortex_line(self._ctx, ctypes.byref(vl), float(scal)))
+
     def random_phase(self, seed, kmax=0.5):
         """Replace the resident field by the random-phase state built on the
         device (the state ic.random_phase builds on the host: unit modes

@@ -161,6 +161,20 @@ typedef struct gpe_zero_point {
   double x, y, z;    /* the point */
 } gpe_zero_point;
 int gpe_sim_zeros(gpe_ctx *ctx, int32_t which, gpe_zero_point *out, int64_t cap, int64_t *count);
+/* A vortex line (type line_param, parameters.in:33-43). */
+typedef struct gpe_line_param {
+  double x0, y0, z0;      /* position; the coordinate along dir is the origin of the disturbance */
+  double amp1, amp2;      /* disturbance amplitudes (amp1*sin, amp2*cos) */
+  double ll;              /* disturbance wavelength */
+  double sgn;             /* sign of the winding */
+  int32_t dir;            /* direction of the line: 0 'x', 1 'y', 2 'z' */
+  int32_t imprint_phase;  /* nonzero: phase only, no core amplitude */
+} gpe_line_param;
+/* imprint_vortex_line (variables.f90:1166-1180), which gpe.f90:272 applies
+ * after each imaginary-time step: psi = abs(psi) * vortex_line(vl), with
+ * the eqn_params scaling `scal`.  A zero wavelength with a non-zero
+ * amplitude is the reference's emergency stop (GPE_EINVAL). */
+int gpe_sim_imprint_vortex_line(gpe_ctx *ctx, const gpe_line_param *vl, double scal);
 /* Random-phase initial state built on the device, replacing the resident
  * field.  The reference's random_phase (ic.f90:195-274) gives the modes
  * inside a sphere of wavenumbers a fixed amplitude and a random phase and

@@ -26,6 +26,8 @@
  *   linelength        variables.f90:896-1162
  *   zeros             io.f90:886-1393 (get_zeros, get_extra_zeros,
  *                     get_re_im_zeros)
+ *   imprint_vortex_line  variables.f90:1166-1180 with vortex_line, amp,
+ *                     get_theta, get_r (ic.f90:314-353, 735-889)
  *
  * Complex arithmetic follows what this image's flang (the compiler the
  * reference is built with here) emits for the reference's expressions,
@@ -979,4 +981,66 @@ long FN(zeros)(const orc_params *p, const REAL *f, int which, orc_zero *out, lon
 #undef ZIM
   ctx_free(&c);
   return n;
+}
+
+/* ---- imprint_vortex_line (variables.f90:1166-1180) ---------------------
+ * psi = abs(psi) * vortex_line(vl): vortex_line (ic.f90:332-353) is
+ * exp(i theta), times amp(r) = 1 - exp(-0.7 r**1.15) (ic.f90:314-328) unless
+ * vl%imprint_phase, with theta from get_theta (ic.f90:824-889) and r from
+ * get_r (ic.f90:735-800) for a line along `dir` displaced by
+ * (a1 sin, a2 cos)(2 pi (c - c0)/ll).  With ll ~ 0 and both amplitudes ~ 0
+ * the wavelength becomes 1 (safe1, safe2); ll ~ 0 with an amplitude is the
+ * reference's emergency_stop (returns -1 here). */
+static REAL r_sin(REAL v) { return IS_FLOAT ? (REAL)sinf((float)v) : (REAL)sin((double)v); }
+static REAL r_cos(REAL v) { return IS_FLOAT ? (REAL)cosf((float)v) : (REAL)cos((double)v); }
+static REAL r_exp(REAL v) { return IS_FLOAT ? (REAL)expf((float)v) : (REAL)exp((double)v); }
+static REAL r_atan2(REAL a, REAL b) { return IS_FLOAT ? (REAL)atan2f((float)a, (float)b) : (REAL)atan2((double)a, (double)b); }
+
+int FN(imprint_vortex_line)(const orc_params *p, REAL *f, const orc_line *vl, double scal_d) {
+  const REAL eps = IS_FLOAT ? (REAL)1.1920929e-07f : (REAL)2.220446049250313e-16;
+  const REAL x0 = (REAL)vl->x0, y0 = (REAL)vl->y0, z0 = (REAL)vl->z0, a1 = (REAL)vl->amp1, a2 = (REAL)vl->amp2;
+  const REAL ll = (REAL)vl->ll, sgn = (REAL)vl->sgn, scal = (REAL)scal_d;
+  REAL safe1 = (REAL)0.0, safe2 = (REAL)0.0;
+  if (r_abs(ll) < eps) {
+    if (r_abs(a1) < eps) safe1 = (REAL)1.0; else return -1;
+    if (r_abs(a2) < eps) safe2 = (REAL)1.0; else return -1;
+  }
+  if (vl->dir < 0 || vl->dir > 2) return -1;
+  const REAL twopi = (REAL)2.0 * (REAL)3.14159265358979323846;  /* 2.0_pr*pi (parameters.f90:124) */
+  const REAL c1 = (REAL)-0.7, c2 = (REAL)1.15;
+  ctx c;
+  ctx_init(&c, p);
+  for (int k = 0; k < p->nz; k++)
+    for (int j = 0; j < p->ny; j++)
+      for (int i = 0; i < p->nx; i++) {
+        const REAL x = c.x[i], y = c.y[j], z = c.z[k];
+        /* (A, B, the coordinate along the line and its origin) per dir */
+        REAL A, B;
+        if (vl->dir == 0) {        /* 'x': atan2(z.., y..) */
+          const REAL t = x - x0;
+          A = y - y0 - a1 * r_sin(twopi * t / (ll + safe1));
+          B = z - z0 - a2 * r_cos(twopi * t / (ll + safe2));
+        } else if (vl->dir == 1) { /* 'y': atan2(x.., z..) */
+          const REAL t = y - y0;
+          A = z - z0 - a1 * r_sin(twopi * t / (ll + safe1));
+          B = x - x0 - a2 * r_cos(twopi * t / (ll + safe2));
+        } else {                   /* 'z': atan2(y.., x..) */
+          const REAL t = z - z0;
+          A = x - x0 - a1 * r_sin(twopi * t / (ll + safe1));
+          B = y - y0 - a2 * r_cos(twopi * t / (ll + safe2));
+        }
+        const REAL theta = sgn * r_atan2(scal * B, scal * A);
+        REAL vr = r_cos(theta), vi = r_sin(theta);
+        if (!vl->imprint_phase) {
+          const REAL r = r_sqrt(scal * (A * A) + scal * (B * B));
+          const REAL am = (REAL)1.0 - r_exp(c1 * r_pow(r, c2));
+          vr = am * vr;
+          vi = am * vi;
+        }
+        const long s = ((long)k * p->ny + j) * p->nx + i;
+        const REAL m = cx_absv(ld(f, s));
+        st(f, s, cx_make(m * vr, m * vi));
+      }
+  ctx_free(&c);
+  return 0;
 }

@@ -51,6 +51,13 @@ typedef struct orc_zero {
   double x, y, z;
 } orc_zero;
 
+/* A vortex line (type line_param, parameters.in:33-43); dir 0/1/2 = x/y/z. */
+typedef struct orc_line {
+  double x0, y0, z0, amp1, amp2, ll, sgn;
+  int dir;
+  int imprint_phase;
+} orc_line;
+
 #define ORC_DECL(P, REAL)                                                     \
   void P##_grid(const orc_params *p, REAL *x, REAL *y, REAL *z);             \
   void P##_laplacian(const orc_params *p, const REAL *f, REAL *lap);        \
@@ -66,7 +73,9 @@ typedef struct orc_zero {
   void P##_renormalise(const orc_params *p, REAL *f, double nn, double norm); \
   double P##_linelength(const orc_params *p, const REAL *f);                \
   long P##_zeros(const orc_params *p, const REAL *f, int which,             \
-                 orc_zero *out, long cap);
+                 orc_zero *out, long cap);                                   \
+  int P##_imprint_vortex_line(const orc_params *p, REAL *f,                 \
+                              const orc_line *vl, double scal);
 
 ORC_DECL(orc64, double)
 ORC_DECL(orc32, float)

@@ -66,6 +66,8 @@ def lib():
                 f = getattr(_LIB, pfx + nm); f.argtypes = [P, vp]; f.restype = D
             f = getattr(_LIB, pfx + "_renormalise"); f.argtypes = [P, vp, D, D]; f.restype = None
             f = getattr(_LIB, pfx + "_zeros"); f.argtypes = [P, vp, I, vp, ctypes.c_long]; f.restype = ctypes.c_long
+            f = getattr(_LIB, pfx + "_imprint_vortex_line"); f.argtypes = [P, vp, ctypes.POINTER(OrcLine), D]
+            f.restype = I
     return _LIB
 
 
@@ -175,6 +177,28 @@ def mass(cfg, psi, nthreads=1):
 def renormalise(cfg, psi, nn, nrm):
     out = np.ascontiguousarray(psi).copy()
     getattr(lib(), _pfx(out) + "_renormalise")(ctypes.byref(params(cfg, out.shape)), _ptr(out), float(nn), float(nrm))
+    return out
+
+
+class OrcLine(ctypes.Structure):
+    """struct orc_line (type line_param, parameters.in:33-43)."""
+    _fields_ = [("x0", ctypes.c_double), ("y0", ctypes.c_double), ("z0", ctypes.c_double),
+                ("amp1", ctypes.c_double), ("amp2", ctypes.c_double), ("ll", ctypes.c_double),
+                ("sgn", ctypes.c_double), ("dir", ctypes.c_int), ("imprint_phase", ctypes.c_int)]
+
+
+def imprint_vortex_line(cfg, psi, line, scal=1.0):
+    """imprint_vortex_line (variables.f90:1166-1180) of a copy of psi; line
+    is a dict with x0, y0, z0, amp1, amp2, ll, sgn, dir ('x'|'y'|'z'),
+    imprint_phase."""
+    out = np.array(psi, copy=True, order="C")
+    vl = OrcLine(float(line.get("x0", 0.0)), float(line.get("y0", 0.0)), float(line.get("z0", 0.0)),
+                 float(line.get("amp1", 0.0)), float(line.get("amp2", 0.0)), float(line.get("ll", 0.0)),
+                 float(line.get("sgn", 1.0)), "xyz".index(line.get("dir", "z")), int(bool(line.get("imprint_phase", True))))
+    rc = getattr(lib(), _pfx(out) + "_imprint_vortex_line")(ctypes.byref(params(cfg, out.shape)), _ptr(out),
+                                                            ctypes.byref(vl), float(scal))
+    if rc != 0:
+        raise ValueError("vortex line disturbance wavelength is zero, but amplitude is non-zero")
     return out
 
 
