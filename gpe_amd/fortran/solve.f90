@@ -93,6 +93,10 @@ module solve
       integer(c_int64_t), intent(out) :: h2d, d2h
       integer(c_int) :: rc
     end function gpe_transfer_bytes
+    subroutine gpe_ctx_destroy(ctx) bind(C, name='gpe_ctx_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine gpe_ctx_destroy
     function gpe_last_error() bind(C, name='gpe_last_error') result(msg)
       import :: c_ptr
       type(c_ptr) :: msg
@@ -102,6 +106,7 @@ module solve
   integer(c_int32_t), parameter :: step_in_resident = 1, step_out_deferred = 2  ! include/gpe_hip.h
 
   type(c_ptr), save :: ctx = c_null_ptr
+  logical, save :: ctx_real_time = .true.  ! real_time the context was made for
   ! device-authoritative mode
   logical, save :: resident = .false.   ! GPE_RESIDENT=1 at the first call
   logical, save :: dev_holds = .false.  ! the context holds the current field
@@ -126,6 +131,15 @@ module solve
     integer(c_int32_t) :: flags
     logical :: use_res
 
+    ! gpe.f90 switches from imaginary to real time once the norm has
+    ! converged (gpe.f90:250-264); the rk45 control of a context follows the
+    ! real_time it was created with, so a switch makes a new one
+    if (c_associated(ctx) .and. (real_time .neqv. ctx_real_time)) then
+      call gpe_ctx_destroy(ctx)
+      ctx = c_null_ptr
+      dev_holds = .false.
+      host_stale = .false.
+    end if
     if (.not. c_associated(ctx)) call create_context()
 
     use_res = resident .and. real_time .and. .not. diagnostic
@@ -350,6 +364,7 @@ module solve
 
     rc = gpe_ctx_create(prm, ctx)
     if (rc /= 0) call emergency_stop('ERROR: '//last_error())
+    ctx_real_time = real_time
 
     call get_environment_variable('GPE_RESIDENT', flag, status=st)
     resident = st == 0 .and. trim(flag) == '1'

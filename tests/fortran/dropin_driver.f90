@@ -19,7 +19,10 @@
 !
 ! Environment (test hooks): DRIVER_STOP_AT=p deletes RUNNING at the top of
 ! iteration p (a user stopping the run); DRIVER_SAVE_AT=p creates SAVE just
-! before solver() in iteration p.
+! before solver() in iteration p; DRIVER_REAL_AT=p switches an
+! imaginary-time run to real time at the top of iteration p the way
+! gpe.f90:250-264 does once the norm has converged (real_time = .true.,
+! im_t = 0, dt = (|Im dt|, |Re dt|)).
 !
 ! Usage: dropin_driver <run.in> <state_in> <state_out>
 program dropin_driver
@@ -30,7 +33,7 @@ program dropin_driver
 
   complex (pr), allocatable :: old(:,:,:), new(:,:,:)
   character(len=512) :: fin_nml, fin, fout
-  integer :: u, io, nx_in, ny_in, nz_in, p_in, n, m, ps, stop_at, save_at, ios
+  integer :: u, io, nx_in, ny_in, nz_in, p_in, n, m, ps, stop_at, save_at, real_at, ios
   real (pr) :: t_in, im_t_in, norm
   complex (pr) :: dt_in
   logical :: run_exist, save_exist, hooked
@@ -54,6 +57,7 @@ program dropin_driver
   close (u)
   stop_at = env_int('DRIVER_STOP_AT')
   save_at = env_int('DRIVER_SAVE_AT')
+  real_at = env_int('DRIVER_REAL_AT')
 
   myrank = 0
   js = 0; je = ny-1; ks = 0; ke = nz-1
@@ -82,6 +86,11 @@ program dropin_driver
   m = int(t/save_rate3)
   old = 0.0_pr
   do while (t+im_t <= end_time)
+    if (p == real_at .and. .not. real_time) then
+      real_time = .true.
+      im_t = 0.0_pr
+      dt = cmplx(abs(aimag(dt)), abs(real(dt, pr)), pr)
+    end if
     if (p == stop_at) then
       open (99, file='RUNNING')
       close (99, status='delete')

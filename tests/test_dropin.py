@@ -189,3 +189,49 @@ def test_resident_dropin_fp32(tmp_path):
     assert (dp / "io.bin").read_bytes() == (dr / "io.bin").read_bytes()
     assert (dp / "out.dat").read_bytes() == (dr / "out.dat").read_bytes()
     assert h2d == cfg.nx * cfg.ny * cfg.nz * 8
+
+
+@pytest.mark.gpu
+@need_bins
+@pytest.mark.parametrize("scheme", ["rk4", "rk45"])
+def test_dropin_imaginary_then_real_time(tmp_path, oracle, scheme):
+    """gpe.f90 switches an imaginary-time run to real time mid-run; the
+    drop-in must follow (rk45 shrinks dt along the real axis afterwards)."""
+    if scheme == "rk45":
+        # steps grow in imaginary time, so real time meets rejections (eps is
+        # the reference's compile-time 1e-6, parameters.f90:90)
+        cfg = cfg_for(scheme=scheme, bcs=2, real_time=False, tau=0.1)
+        psi = np.full(cfg.shape, 0.9, dtype=np.complex128)
+        k, n = 5, 8
+    else:
+        cfg = cfg_for(scheme=scheme, bcs=2, real_time=False, tau=0.01)
+        psi = ic.vortex_ring(cfg, 1.5) * (1 + 0.02 * np.random.default_rng(8).standard_normal(cfg.shape))
+        k, n = 2, 20
+    print("[case] drop-in imaginary -> real", scheme, flush=True)
+    d, _, _ = _run_mode(tmp_path, BIN_D, cfg, psi, n, None, False, DRIVER_REAL_AT=k)
+    got, p, t, im_t, dt = read_state(str(d / "out.dat"))
+    # the driver's loop on the oracle: switch at the top of iteration k
+    st = oracle.make_state(cfg.dt)
+    c, want, steps, real_rejects = cfg, psi, 0, 0
+    while st.t + st.im_t <= (n - 0.5) * cfg.tau:
+        if steps == k and not c.real_time:
+            c = cfg.with_(real_time=True)
+            st.im_t = 0.0
+            st.dt_re, st.dt_im = abs(st.dt_im), abs(st.dt_re)
+        want = oracle.solver(c, st, want)
+        steps += 1
+        if c.real_time:
+            real_rejects += st.n_rejects
+    assert c.real_time and steps > k
+    assert scheme != "rk45" or real_rejects > 0  # the shrink direction is exercised
+    assert p == steps
+    assert dt.imag == 0.0 and dt.real > 0
+    # rk4 follows the oracle to rounding.  rk45 at these large steps feeds
+    # the error estimate -- a difference of nearly equal Cash-Karp sums,
+    # relative rounding ~1e-10 -- back into dt step after step, so the
+    # one-ulp differences of the first steps grow to ~1e-7 by the end; with
+    # the context's old real_time the first real-time rejection would turn
+    # dt imaginary and stall t, which the step count and dt checks above see
+    tol = 1e-12 if scheme == "rk4" else 1e-5
+    assert t == pytest.approx(st.t, rel=tol)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < tol
