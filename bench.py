@@ -218,7 +218,7 @@ def main():
                     help="run only the batch leg and print its JSON (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     from gpe_amd import _lib, ic

@@ -53,7 +53,7 @@ const char *kind_name(int k) {
 // Fields read + written per site by each kind in the fused-sweep traffic
 // model (units of one stored complex): RK4 13 passes per step, Cash-Karp 27
 // per attempt (the error scale is counted as register-resident, so the
-// scale field this build keeps is traffic above the model).  The compensated
+// scale field FP64 and plain FP32 keep is traffic above the model).  The compensated
 // low word (read + write in the final stage) is added by the caller.
 int kind_passes(int k) {
   switch (k) {

@@ -1,8 +1,8 @@
 /*
  * gpe_oracle.c -- CPU restatement of the reference's GP time-stepping path.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, never part of
- * the product.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * TEST INFRASTRUCTURE ONLY: the tests compare the HIP library with it; it is
+ * never part of the product.  Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg load it.
  *
  * Compiled twice (see oracle/Makefile): -DREAL=double -DPFX=orc64 and

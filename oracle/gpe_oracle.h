@@ -1,8 +1,8 @@
 /*
  * gpe_oracle.h -- CPU restatement of the reference's GP time-stepping path.
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker for the HIP library in
- * gpe_amd/: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * TEST INFRASTRUCTURE ONLY.  The tests compare the HIP library in gpe_amd/
+ * with it: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
  * leg may load it.  Nothing in the product links it or falls back to it.
  *
  * Two builds of the same source: REAL=double (symbols orc64_*) restates the

@@ -1,7 +1,7 @@
 """ctypes wrapper of the C oracle (oracle/gpe_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg use it as the checker.  The product (gpe_amd/) never
+cpu_baseline leg use it, to compare against.  The product (gpe_amd/) never
 imports it.
 
 Fields are numpy arrays of shape (nz, ny, nx), C-contiguous, so x is fastest

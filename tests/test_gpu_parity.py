@@ -234,12 +234,12 @@ def test_fp32_forms_track_fp64_rk45(gpu, oracle, precision):
     assert same >= 8
 
 
-def test_config0_ring_rk4_1000_steps(gpu, oracle):
+def test_ring_64cube_rk4_1000_steps(gpu, oracle):
     # 64^3, dx = 0.4, reflective, 2nd order, RK4 dt = 0.01, 1000 steps, V = 0
     cfg = RunConfig(nx=64, ny=64, nz=64, xr=xr_for_spacing(0.4, 64), bcs=2, order=2, scheme="rk4",
                     tau=0.01, g=1.0, mu=1.0)
     psi = ic.vortex_ring(cfg, 4.0)
-    say("config 0: 64^3 ring rk4 x1000")
+    say("64^3 ring rk4 x1000")
     from gpe_amd.sim import Simulation
     st = oracle.make_state(cfg.dt)
     want, wnorms = oracle.run(cfg, st, psi, 1000, NT, with_norms=True)
@@ -248,7 +248,7 @@ def test_config0_ring_rk4_1000_steps(gpu, oracle):
         got = sim.download()
         ge = sim.energy()
     err = rel_l2(got, want)
-    say("config 0 rel L2 = %.3e" % err)
+    say("64^3 ring rel L2 = %.3e" % err)
     assert err < 1e-12
     assert abs(gn[-1] - wnorms[-1]) / wnorms[-1] < 1e-12
     assert ge == pytest.approx(oracle.energy(cfg, want, NT), rel=1e-12)
@@ -260,7 +260,7 @@ def test_config0_ring_rk4_1000_steps(gpu, oracle):
         got32 = sim.download(np.complex128)
         z32 = sim.zeros()
     err32 = rel_l2(got32, want)
-    say("config 0 fp32c rel L2 = %.3e" % err32)
+    say("64^3 ring fp32c rel L2 = %.3e" % err32)
     assert err32 < 1e-5
     # vortex-core positions after 1000 steps (the reference's zeros file):
     # the GPU's FP64 field and the oracle's give the same core points to
@@ -271,19 +271,19 @@ def test_config0_ring_rk4_1000_steps(gpu, oracle):
     zo = oracle.zeros(cfg, want, 0)
     d64 = core_offsets(z64, zo, cfg.dx)
     d32 = core_offsets(z32, zo, cfg.dx)
-    say("config 0 core offsets fp64 %s fp32c %s" % (d64, d32))
+    say("64^3 ring core offsets fp64 %s fp32c %s" % (d64, d32))
     assert d64["points"][0] == d64["points"][1] > 50
     assert d64["max_dx"] < 1e-9
     assert d32["mean_dx"] < 1e-4 and d32["frac_over_tenth_dx"] == 0.0
 
 
-def test_config1_line_pair_256(gpu, oracle):
+def test_line_pair_256cube(gpu, oracle):
     # 256^3, dx = 0.25, 4th order, reflective, trap, RK4 dt = 0.005 (a few steps)
     cfg = RunConfig(nx=256, ny=256, nz=256, xr=xr_for_spacing(0.25, 256), bcs=2, order=4, scheme="rk4",
                     tau=0.005, g=1.0, mu=1.0, omx=0.1, omy=0.1, omz=float(np.sqrt(0.02)))
     psi = ic.line_pair(cfg)
     n = 4
-    say("config 1: 256^3 line pair rk4 x%d" % n)
+    say("256^3 line pair rk4 x%d" % n)
     from gpe_amd.sim import Simulation
     want, _ = oracle.run(cfg, oracle.make_state(cfg.dt), psi, n, NT)
     with Simulation(cfg, psi) as sim:

@@ -20,7 +20,7 @@ pytestmark = [pytest.mark.gpu, pytest.mark.skipif(os.environ.get("GPE_LONG") != 
 NT = min(16, os.cpu_count() or 1)
 
 
-def test_config1_line_pair_256_full(oracle):
+def test_line_pair_256cube_2000_steps(oracle):
     from gpe_amd.sim import Simulation
     n = int(os.environ.get("GPE_LONG_STEPS", "2000"))
     cfg = RunConfig(nx=256, ny=256, nz=256, xr=xr_for_spacing(0.25, 256), bcs=2, order=4, scheme="rk4",
