@@ -1087,7 +1087,11 @@ int gpe_autotune(gpe_ctx *c, int32_t reps) {
       A.kout = c->k[4];
       B.kout = c->k[4];
     }
-    const int zcs[] = {0, 32, 64, 128, 256};
+    // z-chunks to try; the whole z extent gives one chunk per tile, which
+    // on a large grid is a single wave of workgroups (no halo reloads
+    // between waves)
+    std::vector<int> zcs = {0, 32, 64, 128, 256, 512};
+    if (c->P.nz > 512) zcs.push_back(c->P.nz);
     const int nv = c->prec == GPE_FP64 ? N_PAIR_VARIANTS_F64 : N_PAIR_VARIANTS;
     float best = 1e30f;
     int bv = c->pvid[pair], bz = c->pzc[pair];
@@ -1136,7 +1140,8 @@ int gpe_autotune(gpe_ctx *c, int32_t reps) {
     if (kind == K_RK4_2) A.kout = c->k[1];
     if (kind == K_RK4_3) { A.kin[0] = c->k[1]; A.kout = c->k[0]; }
     for (int m = 0; m < 5; ++m) A.b[m] = 0.1;
-    const int zcs[] = {0, 16, 32, 64, 128, 256};
+    std::vector<int> zcs = {0, 16, 32, 64, 128, 256, 512};  // as for the pairs
+    if (c->P.nz > 512) zcs.push_back(c->P.nz);
     float best = 1e30f;
     int bv = c->vid[kind], bz = c->zc[kind];
     for (int v = 0; v < N_TILE_VARIANTS && rc == GPE_OK; ++v) {
