@@ -16,6 +16,7 @@
 
 #include <hipfft/hipfft.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
