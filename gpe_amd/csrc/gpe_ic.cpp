@@ -195,6 +195,9 @@ int random_phase_device(int nx, int ny, int nz, double dx, double dy, double dz,
   IC_CHK(hipStreamSynchronize(s));
 done:
 #undef IC_CHK
+  // on an early exit, let queued copies and kernels finish before their
+  // buffers (host vectors included) go away
+  if (rc) (void)hipStreamSynchronize(s);
   if (have_plan) hipfftDestroy(plan);
   if (d_idx) hipFree(d_idx);
   if (d_v) hipFree(d_v);
