@@ -43,7 +43,7 @@ ZERO_FILES = {"zeros": 0, "re_zeros": 1, "im_zeros": 2}
 # Exported entry points (must match include/gpe_hip.h).
 EXPORTS = [
     "gpe_last_error", "gpe_ctx_create", "gpe_ctx_destroy", "gpe_solver_step", "gpe_solver_step_ex",
-    "gpe_transfer_bytes", "gpe_sim_imprint_vortex_line", "gpe_sim_random_phase", "gpe_random_phase_modes", "gpe_batch_add_random_phase",
+    "gpe_transfer_bytes", "gpe_sim_imprint_vortex_line", "gpe_sim_random_phase", "gpe_random_phase_prepare", "gpe_random_phase_modes", "gpe_batch_add_random_phase",
     "gpe_sim_upload", "gpe_sim_download", "gpe_sim_set_state", "gpe_sim_get_state",
     "gpe_sim_advance", "gpe_sim_norm_history", "gpe_sim_norm", "gpe_sim_energy",
     "gpe_sim_mass", "gpe_sim_linelength", "gpe_sim_zeros", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
@@ -107,6 +107,7 @@ def lib():
         "gpe_solver_step_ex": ([vp, vp, i32, vp, S, i32], ctypes.c_int),
         "gpe_transfer_bytes": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),
         "gpe_sim_random_phase": ([vp, ctypes.c_uint64, ctypes.c_double], ctypes.c_int),
+        "gpe_random_phase_prepare": ([i32, i32, i32], ctypes.c_int),
         "gpe_sim_imprint_vortex_line": ([vp, ctypes.POINTER(GpeLineParam), ctypes.c_double], ctypes.c_int),
         "gpe_random_phase_modes": ([i32, i32, i32] + [ctypes.c_double] * 4 + [ctypes.c_uint64, vp, vp, i64,
                                                                                 ctypes.POINTER(i64)], ctypes.c_int),

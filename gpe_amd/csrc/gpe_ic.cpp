@@ -18,6 +18,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 namespace gpe {
@@ -74,6 +77,33 @@ __global__ void ic_store(const double2 *f, size_t n, double s, int prec, void *o
         static_cast<float2 *>(lo)[q] = make_float2((float)(re - (double)hr), (float)(im - (double)hi));
     }
   }
+}
+
+// Inverse Z2Z plans by grid, created on first use (or ahead of it by
+// gpe_random_phase_prepare, from any thread) and kept for the life of the
+// process: creating a plan costs far more than running it, and a served
+// request stream brings the same few grid sizes again and again.  The
+// mutex also serialises set-stream + execute on a shared plan.
+std::mutex g_plan_mu;
+std::map<std::tuple<int, int, int>, hipfftHandle> g_plans;
+
+const char *fft_err(hipfftResult r);
+
+// A cached plan for the (nz, ny, nx) grid; call with g_plan_mu held.
+int plan_for(int nx, int ny, int nz, hipfftHandle *out) {
+  const auto key = std::make_tuple(nx, ny, nz);
+  auto it = g_plans.find(key);
+  if (it != g_plans.end()) {
+    *out = it->second;
+    return GPE_OK;
+  }
+  hipfftHandle plan = 0;
+  // (nz, ny, nx) with x fastest: hipFFT's first size is the slowest
+  const hipfftResult fr = hipfftPlan3d(&plan, nz, ny, nx, HIPFFT_Z2Z);
+  if (fr != HIPFFT_SUCCESS) return fail(GPE_EHIP, "random phase: hipfftPlan3d: %s", fft_err(fr));
+  g_plans[key] = plan;
+  *out = plan;
+  return GPE_OK;
 }
 
 const char *fft_err(hipfftResult r) {
@@ -146,8 +176,6 @@ int random_phase_device(int nx, int ny, int nz, double dx, double dy, double dz,
   long long *d_idx = nullptr;
   double2 *d_v = nullptr;
   double *d_part = nullptr;
-  hipfftHandle plan = 0;
-  bool have_plan = false;
   std::vector<double> part(kIcBlocks);
   hipfftResult fr;
   double sum = 0.0;
@@ -167,17 +195,16 @@ int random_phase_device(int nx, int ny, int nz, double dx, double dy, double dz,
   IC_CHK(hipMemsetAsync(f, 0, n * sizeof(double2), s));
   hipLaunchKernelGGL(ic_scatter, dim3((nm + 255) / 256), dim3(256), 0, s, f, d_idx, d_v, nm);
   IC_CHK(hipGetLastError());
-  // (nz, ny, nx) with x fastest: hipFFT's first size is the slowest
-  if ((fr = hipfftPlan3d(&plan, nz, ny, nx, HIPFFT_Z2Z)) != HIPFFT_SUCCESS) {
-    rc = fail(GPE_EHIP, "random phase: hipfftPlan3d: %s", fft_err(fr));
-    goto done;
-  }
-  have_plan = true;
-  if ((fr = hipfftSetStream(plan, s)) != HIPFFT_SUCCESS ||
-      (fr = hipfftExecZ2Z(plan, (hipfftDoubleComplex *)f, (hipfftDoubleComplex *)f, HIPFFT_BACKWARD)) !=
-          HIPFFT_SUCCESS) {
-    rc = fail(GPE_EHIP, "random phase: inverse FFT: %s", fft_err(fr));
-    goto done;
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    hipfftHandle plan = 0;
+    if ((rc = plan_for(nx, ny, nz, &plan))) goto done;
+    if ((fr = hipfftSetStream(plan, s)) != HIPFFT_SUCCESS ||
+        (fr = hipfftExecZ2Z(plan, (hipfftDoubleComplex *)f, (hipfftDoubleComplex *)f, HIPFFT_BACKWARD)) !=
+            HIPFFT_SUCCESS) {
+      rc = fail(GPE_EHIP, "random phase: inverse FFT: %s", fft_err(fr));
+      goto done;
+    }
   }
   hipLaunchKernelGGL(ic_sumsq, dim3(kIcBlocks), dim3(256), 0, s, f, n, d_part);
   IC_CHK(hipGetLastError());
@@ -198,7 +225,6 @@ done:
   // on an early exit, let queued copies and kernels finish before their
   // buffers (host vectors included) go away
   if (rc) (void)hipStreamSynchronize(s);
-  if (have_plan) hipfftDestroy(plan);
   if (d_idx) hipFree(d_idx);
   if (d_v) hipFree(d_v);
   if (d_part) hipFree(d_part);
@@ -223,4 +249,11 @@ extern "C" int gpe_random_phase_modes(int32_t nx, int32_t ny, int32_t nz, double
     phasor[2 * q + 1] = im[q];
   }
   return GPE_OK;
+}
+
+extern "C" int gpe_random_phase_prepare(int32_t nx, int32_t ny, int32_t nz) {
+  if (nx < 1 || ny < 1 || nz < 1) return gpe::fail(GPE_EINVAL, "random phase: bad grid");
+  std::lock_guard<std::mutex> lk(gpe::g_plan_mu);
+  hipfftHandle plan = 0;
+  return gpe::plan_for(nx, ny, nz, &plan);
 }

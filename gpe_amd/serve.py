@@ -27,6 +27,7 @@ import numpy as np
 
 from . import batch as _batch
 from .batch import Batch
+from .sim import random_phase_prepare
 from .workloads import Request
 
 
@@ -110,8 +111,11 @@ class Server:
 
         def prefetch():
             for r in list(queue)[: self.prefetch]:
-                if r.rid not in futures and not on_device(r):
-                    futures[r.rid] = self.pool.submit(r.initial_state)
+                if r.rid not in futures:
+                    # host states are built ahead; for device-built ones the
+                    # FFT plan of the grid is made ahead
+                    job = (lambda c=r.cfg: random_phase_prepare(c)) if on_device(r) else r.initial_state
+                    futures[r.rid] = self.pool.submit(job)
 
         while True:
             now = clock()
@@ -128,6 +132,9 @@ class Server:
                     raise ValueError("request %d can never fit the HBM budget" % r.rid)
                 if on_device(r):
                     # the random-phase state is built in the instance's memory
+                    f = futures.pop(r.rid, None)
+                    if f is not None:
+                        f.result()  # its FFT plan is ready
                     iid = r.add_to(self.batch, device_ic=True)
                     if iid is None:  # does not fit now: wait for memory
                         break

@@ -254,6 +254,12 @@ class Simulation:
         return self.tune_get()
 
 
+def random_phase_prepare(cfg: RunConfig):
+    """Create the cached inverse-FFT plan the device random-phase state of
+    this grid uses (safe from a worker thread; no field is touched)."""
+    _lib.check(_lib.lib().gpe_random_phase_prepare(int(cfg.nx), int(cfg.ny), int(cfg.nz)))
+
+
 def random_phase_modes(cfg: RunConfig, seed, kmax=0.5):
     """Flat indices and unit phasors of the random-phase state's modes, as
     the device generator forms them (host only, no GPU work)."""

@@ -185,6 +185,11 @@ int gpe_sim_imprint_vortex_line(gpe_ctx *ctx, const gpe_line_param *vl, double s
  * index; inverse FFT (hipFFT); rescaled so <|psi|^2> = 1; converted to the
  * context's storage form. */
 int gpe_sim_random_phase(gpe_ctx *ctx, uint64_t seed, double kmax);
+/* Create, ahead of use and from any thread, the cached inverse-FFT plan
+ * that gpe_sim_random_phase and gpe_batch_add_random_phase use for this
+ * grid (plans are kept for the life of the process; creating one costs far
+ * more than running it). */
+int gpe_random_phase_prepare(int32_t nx, int32_t ny, int32_t nz);
 /* The mode list of that state, computed on the host (no device work):
  * writes min(count, cap) flat indices (i + nx*(j + ny*k)) and unit phasors
  * (re, im pairs) and sets *count. */
