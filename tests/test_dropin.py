@@ -235,3 +235,21 @@ def test_dropin_imaginary_then_real_time(tmp_path, oracle, scheme):
     tol = 1e-12 if scheme == "rk4" else 1e-5
     assert t == pytest.approx(st.t, rel=tol)
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < tol
+
+
+@pytest.mark.gpu
+@need_bins
+@pytest.mark.parametrize("eqn", [1, 2, 3])
+def test_dropin_other_equations(tmp_path, oracle, eqn):
+    """eqn_to_solve 1-3 with the frame velocity Urhs and (eqn 1, 3) the
+    dissipation of solve.f90:397-411, through the drop-in."""
+    cfg = cfg_for(scheme="rk4", bcs=2, eqn=eqn, urhs=0.3, diss_amp=0.5 if eqn != 2 else 0.0, tau=0.002)
+    psi = ic.vortex_ring(cfg, 1.5) * (1 + 0.02 * np.random.default_rng(9).standard_normal(cfg.shape))
+    n = 6
+    print("[case] drop-in eqn", eqn, flush=True)
+    r = run_dropin(tmp_path, BIN_D, cfg, psi, n)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got, p, t, im_t, dt = read_state(str(tmp_path / "out.dat"))
+    want, _ = oracle.run(cfg, oracle.make_state(cfg.dt), psi, n)
+    assert p == n
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12
