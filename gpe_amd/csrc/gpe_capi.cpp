@@ -575,13 +575,17 @@ int gpe_ctx_create(const gpe_params *params, gpe_ctx **out) {
       c->vid[kk] = 1;
       c->zc[kk] = 0;
     }
+    // every pair runs as its two stages until the autotuner (or
+    // gpe_tune_set with a variant >= 0) finds the fused sweep faster: the
+    // one-stage kernels hold up at every grid size, a fused pair's register
+    // pressure does not (RKF45 at 1024^3: 148 ms per step fused against
+    // 53 ms unfused).  The FP32 forms allow fusion; FP64 (the parity path)
+    // never fuses.
     for (int q = 0; q < N_PAIR_KINDS; ++q) {
       c->pvid[q] = 0;
       c->pzc[q] = 0;
-      c->pfuse[q] = true;
+      c->pfuse[q] = false;
     }
-    // the FP32 forms fuse stage pairs by default; FP64 (the parity path)
-    // runs the stages one by one unless asked
     c->fuse = c->prec != GPE_FP64;
     int cap = 0;
     for (int v = 0; v < N_TILE_VARIANTS; ++v) {

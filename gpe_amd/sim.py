@@ -234,7 +234,9 @@ class Simulation:
         _lib.check(self._L.gpe_tune_set(self._ctx, k, int(variant), int(zchunk)))
 
     def set_fusion(self, enable=True):
-        """Fuse RK4 / Cash-Karp stages pairwise (bitwise the same fields)."""
+        """Allow fused RK4 / Cash-Karp stage pairs (bitwise the same fields);
+        a pair runs fused once its variant is set >= 0 (tune_set or
+        autotune)."""
         _lib.check(self._L.gpe_set_fusion(self._ctx, 1 if enable else 0))
 
     def tune_get(self):

@@ -210,9 +210,10 @@ int gpe_tune_set(gpe_ctx *ctx, int32_t kind, int32_t variant, int32_t zchunk);
 /* Fuse consecutive RK4 / Cash-Karp stages pairwise into one sweep each
  * (RK4: 7 field passes per step instead of 13; Cash-Karp: 14 per attempt
  * instead of 27).  Results are bitwise those of the stage-by-stage sweeps.
- * Default: on for GPE_FP32 / GPE_FP32C (each pair still runs unfused
- * unless its tuned variant is >= 0).  GPE_FP64 has no fused kernels:
- * enable = 1 on a GPE_FP64 context returns GPE_EINVAL. */
+ * Default: allowed for GPE_FP32 / GPE_FP32C, with every pair unfused until
+ * gpe_autotune or gpe_tune_set (variant >= 0) selects its fused sweep.
+ * GPE_FP64 has no fused kernels: enable = 1 on a GPE_FP64 context returns
+ * GPE_EINVAL. */
 int gpe_set_fusion(gpe_ctx *ctx, int32_t enable);
 int gpe_tune_get(gpe_ctx *ctx, int32_t kind, int32_t *variant, int32_t *zchunk, int32_t *waves,
                  int32_t *rows_per_wave);

@@ -40,9 +40,12 @@ def ic_of(cfg):
 
 
 def _run(cfg, psi, n, fused, tune=None):
+    from gpe_amd import _lib
     from gpe_amd.sim import Simulation
     with Simulation(cfg, psi) as sim:
         sim.set_fusion(fused)
+        if fused and not tune:
+            tune = {p: (0, 0) for p in _lib.PAIR_KINDS}  # every pair fused
         if tune:
             for k, (v, z) in tune.items():
                 sim.tune_set(k, v, z)
