@@ -38,12 +38,18 @@ def _ptr(a):
 class Simulation:
     """One GP simulation resident on the GPU."""
 
-    def __init__(self, cfg: RunConfig, psi=None, dt=None, t=0.0, im_t=0.0, p=0):
+    def __init__(self, cfg: RunConfig, psi=None, dt=None, t=0.0, im_t=0.0, p=0, tuned=True):
+        """tuned: apply the stored launch configuration of this problem
+        (gpe_amd/tuning.json or $GPE_TUNING_FILE) when there is one; the
+        launch configuration changes speed only, never the results."""
         self.cfg = cfg
         self._L = _lib.lib()
         self._ctx = ctypes.c_void_p()
         _lib.check(self._L.gpe_ctx_create(ctypes.byref(make_params(cfg)), ctypes.byref(self._ctx)))
         self.set_state(cfg.dt if dt is None else dt, t=t, im_t=im_t, p=p)
+        if tuned:
+            from . import tuning
+            tuning.apply(self, tuning.DEFAULT_PATH)
         if psi is not None:
             self.upload(psi)
 
