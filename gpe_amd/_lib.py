@@ -12,6 +12,7 @@ LIB_PATH = os.environ.get("GPE_LIBRARY") or os.path.join(HERE, "libgpehip.so")
 
 GPE_OK = 0
 GPE_EFULL = -7
+GPE_EIO = -8
 GPE_EULER, GPE_RK2, GPE_RK4, GPE_RK45 = 1, 2, 4, 45
 GPE_FP64, GPE_FP32, GPE_FP32C = 0, 1, 2
 GPE_HOST_C128, GPE_HOST_C64 = 0, 1
@@ -47,7 +48,7 @@ EXPORTS = [
     "gpe_sim_upload", "gpe_sim_download", "gpe_sim_set_state", "gpe_sim_get_state",
     "gpe_sim_advance", "gpe_sim_norm_history", "gpe_sim_norm", "gpe_sim_energy",
     "gpe_sim_mass", "gpe_sim_linelength", "gpe_sim_zeros", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
-    "gpe_tune_set", "gpe_set_fusion", "gpe_tune_get", "gpe_autotune", "gpe_probe_bandwidth",
+    "gpe_tune_set", "gpe_set_fusion", "gpe_tune_get", "gpe_tune_key", "gpe_tune_load", "gpe_autotune", "gpe_probe_bandwidth",
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_footprint", "gpe_batch_add",
     "gpe_batch_remove", "gpe_batch_run", "gpe_batch_sync", "gpe_batch_poll", "gpe_batch_query",
     "gpe_batch_download", "gpe_batch_stats", "gpe_batch_use_graph", "gpe_batch_tune_set",
@@ -131,6 +132,8 @@ def lib():
         "gpe_tune_set": ([vp, i32, i32, i32], ctypes.c_int),
         "gpe_set_fusion": ([vp, i32], ctypes.c_int),
         "gpe_tune_get": ([vp, i32] + [ctypes.POINTER(i32)] * 4, ctypes.c_int),
+        "gpe_tune_key": ([vp, ctypes.c_char_p, i64], ctypes.c_int),
+        "gpe_tune_load": ([vp, ctypes.c_char_p, ctypes.POINTER(i32)], ctypes.c_int),
         "gpe_autotune": ([vp, i32], ctypes.c_int),
         "gpe_probe_bandwidth": ([i64, i32, dp, dp, dp], ctypes.c_int),
         "gpe_batch_create": ([i64, ctypes.POINTER(vp)], ctypes.c_int),

@@ -4,9 +4,11 @@
 // kernels for each scheme (solve.f90:57-378), runs the rk45 step-size
 // control of rkqs (solve.f90:158-244) and replaces the reference's
 // solver(var_in, var_out) call (solve.f90:25-53).
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
@@ -1028,6 +1030,72 @@ int gpe_tune_get(gpe_ctx *c, int32_t kind, int32_t *variant, int32_t *zchunk, in
   if (zchunk) *zchunk = c->zc[kind];
   if (nw) *nw = kTiles[c->vid[kind]].nw;
   if (rows) *rows = kTiles[c->vid[kind]].p;
+  return GPE_OK;
+}
+
+int gpe_tune_key(gpe_ctx *c, char *buf, int64_t cap) {
+  if (!c || !buf) return fail(GPE_EINVAL, "NULL argument");
+  static const char *precs[] = {"fp64", "fp32", "fp32c"};
+  const char *scheme = c->P.scheme == GPE_EULER ? "euler"
+                       : c->P.scheme == GPE_RK2 ? "rk2"
+                       : c->P.scheme == GPE_RK4 ? "rk4"
+                                                : "rk45";
+  const int n = std::snprintf(buf, cap > 0 ? static_cast<size_t>(cap) : 0, "%dx%dx%d/o%d/%s/%s/b%d%s", c->P.nx,
+                              c->P.ny, c->P.nz, c->P.order, precs[c->prec], scheme, c->P.bcs,
+                              c->fuse ? "/fused" : "");
+  if (n < 0 || n >= cap) return fail(GPE_EINVAL, "key buffer too small (%d bytes needed)", n + 1);
+  return GPE_OK;
+}
+
+int gpe_tune_load(gpe_ctx *c, const char *path, int32_t *applied) {
+  if (!c) return fail(GPE_EINVAL, "NULL context");
+  if (applied) *applied = 0;
+  std::string file;
+  if (path && *path) {
+    file = path;
+  } else if (const char *env = std::getenv("GPE_TUNING_FILE"); env && *env) {
+    file = env;
+  } else {
+    // tuning.json in the directory of this library
+    Dl_info info{};
+    if (!dladdr(reinterpret_cast<void *>(&gpe_tune_load), &info) || !info.dli_fname)
+      return fail(GPE_EIO, "cannot locate the library to find tuning.json");
+    file = info.dli_fname;
+    const size_t slash = file.rfind('/');
+    file = (slash == std::string::npos ? std::string(".") : file.substr(0, slash)) + "/tuning.json";
+  }
+  char key[128];
+  if (int rc = gpe_tune_key(c, key, sizeof key)) return rc;
+  std::vector<TuneItem> items;
+  bool found = false;
+  if (int rc = read_tune_table(file.c_str(), key, items, found)) return rc;
+  if (!found) return GPE_OK;
+  // names this library does not know (a table from another version) are skipped
+  std::vector<int> kinds(items.size(), -1);
+  for (size_t i = 0; i < items.size(); ++i)
+    for (int k = 0; k < KP0 + N_PAIR_KINDS; ++k)
+      if ((k < K_COUNT || k >= KP0) && items[i].name == kind_name(k)) kinds[i] = k;
+  int vid[K_COUNT], zc[K_COUNT], pvid[N_PAIR_KINDS], pzc[N_PAIR_KINDS];
+  bool pfuse[N_PAIR_KINDS];
+  std::memcpy(vid, c->vid, sizeof vid);
+  std::memcpy(zc, c->zc, sizeof zc);
+  std::memcpy(pvid, c->pvid, sizeof pvid);
+  std::memcpy(pzc, c->pzc, sizeof pzc);
+  std::memcpy(pfuse, c->pfuse, sizeof pfuse);
+  for (size_t i = 0; i < items.size(); ++i) {
+    if (kinds[i] < 0) continue;
+    if (gpe_tune_set(c, kinds[i], items[i].variant, items[i].zchunk) != GPE_OK) {
+      // leave the configuration as it was
+      std::memcpy(c->vid, vid, sizeof vid);
+      std::memcpy(c->zc, zc, sizeof zc);
+      std::memcpy(c->pvid, pvid, sizeof pvid);
+      std::memcpy(c->pzc, pzc, sizeof pzc);
+      std::memcpy(c->pfuse, pfuse, sizeof pfuse);
+      return fail(GPE_EIO, "%s: entry %s, kind %s: %s", file.c_str(), key, items[i].name.c_str(),
+                  g_err.c_str());
+    }
+  }
+  if (applied) *applied = 1;
   return GPE_OK;
 }
 

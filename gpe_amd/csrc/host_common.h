@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 namespace gpe {
 // Record the message of a failed call (read back by gpe_last_error) and
@@ -28,4 +29,14 @@ int field_upload(int prec, size_t plane, int nz, const void *src, int host_type,
                  size_t row_off, void *hi, void *lo);
 int field_download(int prec, size_t plane, int nz, const void *hi, const void *lo, void *dst, int host_type);
 extern thread_local std::string g_err;
+
+// One stored launch configuration: stage-kind name, tile variant, z-chunk.
+struct TuneItem {
+  std::string name;
+  int variant = 0, zchunk = 0;
+};
+// The entry stored under key in the launch-configuration table at path
+// (tune_table.cpp).  A missing file or key leaves found false and returns
+// GPE_OK; an unreadable or malformed file returns an error.
+int read_tune_table(const char *path, const char *key, std::vector<TuneItem> &out, bool &found);
 }  // namespace gpe

@@ -93,6 +93,12 @@ module solve
       integer(c_int64_t), intent(out) :: h2d, d2h
       integer(c_int) :: rc
     end function gpe_transfer_bytes
+    function gpe_tune_load(ctx, path, applied) bind(C, name='gpe_tune_load') result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx, path
+      integer(c_int32_t), intent(out) :: applied
+      integer(c_int) :: rc
+    end function gpe_tune_load
     subroutine gpe_ctx_destroy(ctx) bind(C, name='gpe_ctx_destroy')
       import :: c_ptr
       type(c_ptr), value :: ctx
@@ -312,6 +318,7 @@ module solve
 
     type(gpe_params_t) :: prm
     integer(c_int) :: rc
+    integer(c_int32_t) :: applied
     character(len=8) :: flag
     integer :: st
 
@@ -363,6 +370,10 @@ module solve
     prm%dt_decrease = dt_decrease
 
     rc = gpe_ctx_create(prm, ctx)
+    if (rc /= 0) call emergency_stop('ERROR: '//last_error())
+    ! stored launch configuration of this problem ($GPE_TUNING_FILE, else
+    ! tuning.json beside the library), if any: speed only
+    rc = gpe_tune_load(ctx, c_null_ptr, applied)
     if (rc /= 0) call emergency_stop('ERROR: '//last_error())
     ctx_real_time = real_time
 

@@ -7,6 +7,7 @@ numpy arrays of shape (nz, ny, nx) -- x fastest, the Fortran order of the
 reference's psi(0:nx1, js:je, ks:ke).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -256,6 +257,20 @@ class Simulation:
                                             ctypes.byref(r)))
             out[nm] = (v.value, z.value, w.value, r.value)
         return out
+
+    def tune_key(self):
+        """This problem's key in the launch-configuration table."""
+        buf = ctypes.create_string_buffer(128)
+        _lib.check(self._L.gpe_tune_key(self._ctx, buf, len(buf)))
+        return buf.value.decode()
+
+    def tune_load(self, path=None):
+        """Apply the stored launch configuration of this problem from the
+        table at path (default: $GPE_TUNING_FILE, else tuning.json beside the
+        library); True if the table held one."""
+        got = ctypes.c_int32()
+        _lib.check(self._L.gpe_tune_load(self._ctx, os.fsencode(path) if path else None, ctypes.byref(got)))
+        return bool(got.value)
 
     def autotune(self, reps=3):
         _lib.check(self._L.gpe_autotune(self._ctx, int(reps)))

@@ -2,21 +2,22 @@
 
 The autotuner (gpe_autotune in the C ABI) times every compiled tile variant
 and several z-chunk lengths per stage kind on the resident field.  Results
-are kept in a JSON file keyed by (grid, order, precision, scheme) so a later
-run of the same problem skips the search.
+are kept in a JSON file keyed by (grid, order, precision, scheme, boundary
+condition, fusion) so a later run of the same problem skips the search; the
+library's gpe_tune_load applies an entry to a context (also from C and from
+the Fortran drop-in).
 """
 import json
 import os
-
-from . import _lib
 
 DEFAULT_PATH = os.environ.get("GPE_TUNING_FILE", os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                               "tuning.json"))
 
 
 def key(cfg, fused=None):
-    """fused: whether the stages run fused pairwise (default: the library's
-    default, on for the FP32 forms)."""
+    """The table key of cfg's problem, as gpe_tune_key forms it for a
+    context.  fused: whether the context allows fused stage pairs (default:
+    the library's default, allowed for the FP32 forms)."""
     if fused is None:
         fused = cfg.precision != "fp64"
     return "%dx%dx%d/o%d/%s/%s/b%d%s" % (cfg.nx, cfg.ny, cfg.nz, cfg.order, cfg.precision, cfg.scheme, cfg.bcs,
@@ -38,14 +39,10 @@ def save(table, path=DEFAULT_PATH):
 
 
 def apply(sim, path=DEFAULT_PATH):
-    """Apply a stored configuration for sim's problem; True if one existed."""
-    ent = load(path).get(key(sim.cfg))
-    if not ent:
-        return False
-    for kind, (variant, zc) in ent.items():
-        if kind in _lib.KIND_NAMES:
-            sim.tune_set(kind, variant, zc)
-    return True
+    """Apply a stored configuration for sim's problem; True if one existed.
+    The library reads the table (gpe_tune_load), so C and Fortran callers
+    apply the same entries."""
+    return sim.tune_load(path)
 
 
 def tune(sim, reps=3, path=DEFAULT_PATH, store=True):
@@ -54,6 +51,6 @@ def tune(sim, reps=3, path=DEFAULT_PATH, store=True):
     ent = {k: [v[0], v[1]] for k, v in got.items()}
     if store:
         table = load(path)
-        table[key(sim.cfg)] = ent
+        table[sim.tune_key()] = ent
         save(table, path)
     return ent

@@ -36,6 +36,7 @@ extern "C" {
 #define GPE_ESTEP (-5)    /* adaptive step control gave up (too many rejections) */
 #define GPE_ESTATE (-6)   /* call out of order (e.g. no field uploaded) */
 #define GPE_EFULL (-7)    /* batch: the instance does not fit the HBM budget */
+#define GPE_EIO (-8)      /* a file could not be read or parsed */
 
 /* scheme (run_params%scheme, solve.f90:34-50) */
 #define GPE_EULER 1
@@ -217,6 +218,20 @@ int gpe_tune_set(gpe_ctx *ctx, int32_t kind, int32_t variant, int32_t zchunk);
 int gpe_set_fusion(gpe_ctx *ctx, int32_t enable);
 int gpe_tune_get(gpe_ctx *ctx, int32_t kind, int32_t *variant, int32_t *zchunk, int32_t *waves,
                  int32_t *rows_per_wave);
+/* Stored launch configurations.  The autotuner's picks are kept in a JSON
+ * table keyed by problem, "NXxNYxNZ/oORDER/PREC/SCHEME/bBCS" with "/fused"
+ * appended where the context allows fused pairs (PREC fp64 | fp32 | fp32c,
+ * SCHEME euler | rk2 | rk4 | rk45), each entry mapping stage-kind names
+ * (gpe_prof_query) to [variant, zchunk].  gpe_tune_key writes the context's
+ * key into buf (cap bytes, NUL-terminated).  gpe_tune_load applies the entry
+ * for the context's problem from the table at path; path NULL or "" means
+ * $GPE_TUNING_FILE, else tuning.json beside this library.  A missing table
+ * or key leaves the configuration unchanged and sets *applied = 0 (1 when
+ * an entry was applied); a malformed table returns GPE_EIO.  Speed only:
+ * results do not depend on the launch configuration.  No reference
+ * counterpart. */
+int gpe_tune_key(gpe_ctx *ctx, char *buf, int64_t cap);
+int gpe_tune_load(gpe_ctx *ctx, const char *path, int32_t *applied);
 /* Time every variant and several z-chunks of each stage kind (and, with
  * fusion on, of each stage pair) of the context's scheme on the resident
  * field (reps launches each) and keep the fastest; a pair runs fused only
