@@ -1091,8 +1091,8 @@ int gpe_tune_load(gpe_ctx *c, const char *path, int32_t *applied) {
       std::memcpy(c->pvid, pvid, sizeof pvid);
       std::memcpy(c->pzc, pzc, sizeof pzc);
       std::memcpy(c->pfuse, pfuse, sizeof pfuse);
-      return fail(GPE_EIO, "%s: entry %s, kind %s: %s", file.c_str(), key, items[i].name.c_str(),
-                  g_err.c_str());
+      const std::string why = g_err;
+      return fail(GPE_EIO, "%s: entry %s, kind %s: %s", file.c_str(), key, items[i].name.c_str(), why.c_str());
     }
   }
   if (applied) *applied = 1;
