@@ -227,9 +227,10 @@ int gpe_tune_get(gpe_ctx *ctx, int32_t kind, int32_t *variant, int32_t *zchunk, 
  * for the context's problem from the table at path; path NULL or "" means
  * $GPE_TUNING_FILE, else tuning.json beside this library.  A missing table
  * or key leaves the configuration unchanged and sets *applied = 0 (1 when
- * an entry was applied); a malformed table returns GPE_EIO.  Speed only:
- * results do not depend on the launch configuration.  No reference
- * counterpart. */
+ * an entry was applied).  Kind names this library does not know are
+ * skipped.  A malformed table, or an entry gpe_tune_set would reject,
+ * returns GPE_EIO with the configuration unchanged.  Speed only: results
+ * do not depend on the launch configuration.  No reference counterpart. */
 int gpe_tune_key(gpe_ctx *ctx, char *buf, int64_t cap);
 int gpe_tune_load(gpe_ctx *ctx, const char *path, int32_t *applied);
 /* Time every variant and several z-chunks of each stage kind (and, with
