@@ -29,7 +29,8 @@ UNITS = [("gpe_misc.hip", "gpe_misc", ["-ffp-contract=off"]),
          ("gpe_batch.cpp", "gpe_batch", ["-ffp-contract=off", "-x", "hip"]),
          ("host_common.cpp", "host_common", ["-ffp-contract=off", "-x", "hip"]),
          ("gpe_ic.cpp", "gpe_ic", ["-ffp-contract=off", "-x", "hip"]),
-         ("tune_table.cpp", "tune_table", ["-x", "hip"])]
+         ("tune_table.cpp", "tune_table", ["-x", "hip"]),
+         ("gpe_spectral.cpp", "gpe_spectral", ["-ffp-contract=off", "-x", "hip"])]
 for _v in range(N_VARIANTS):
     # stage outputs are written with nontemporal stores (-DGPE_NT_STORE): they
     # are not re-read within a sweep, and keeping them out of L2 leaves room

@@ -21,6 +21,7 @@
 #include "../../include/gpe_hip.h"
 #include "host_common.h"
 #include "gpe_ic.h"
+#include "gpe_spectral.h"
 #include "launch.h"
 
 using namespace gpe;
@@ -137,6 +138,8 @@ struct gpe_ctx {
   int ll_cap = 0;
   void *zr_buf = nullptr;  // zeros pass: cap points + a counter
   int zr_cap = 0;
+  // FFT-side diagnostics: a complex128 field and small scratch (on demand)
+  void *spec = nullptr, *spec_aux = nullptr;
   double prof_ms[GPE_PROF_KINDS] = {};
 };
 
@@ -487,7 +490,7 @@ int solver_call(gpe_ctx *c, double *hist_slot) {
 
 void free_all(gpe_ctx *c) {
   void *ptrs[] = {c->psi, c->psi2, c->lo, c->lo2, c->k[0], c->k[1], c->k[2], c->k[3], c->k[4], c->k[5],
-                  c->acc, c->scale, c->part, c->dscal, c->hist, c->ll_buf, c->zr_buf};
+                  c->acc, c->scale, c->part, c->dscal, c->hist, c->ll_buf, c->zr_buf, c->spec, c->spec_aux};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->hscal) hipHostFree(c->hscal);
@@ -768,9 +771,26 @@ double ll_singles(std::vector<LinePoint> &w, std::vector<LinePoint> &s) {
 
 extern "C" {
 
+}  // extern "C"
+
+namespace {
+int linelength_of(gpe_ctx *c, int prec, const void *psi, const void *lo, double *length);
+}  // namespace
+
+extern "C" {
+
 int gpe_sim_linelength(gpe_ctx *c, double *length) {
   if (!c || !length) return fail(GPE_EINVAL, "NULL argument");
   if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  return linelength_of(c, c->prec, c->psi, c->comp ? c->lo : nullptr, length);
+}
+
+}  // extern "C"
+
+namespace {
+// linelength (variables.f90:896-1162) of a device field of the context's
+// grid in storage form prec (lo: the compensated low word or NULL).
+int linelength_of(gpe_ctx *c, int prec, const void *psi, const void *lo, double *length) {
   const long long ncell = (long long)(c->P.nx - 2) * (c->P.ny - 2) * (c->P.nz - 2);
   const int nb = (int)std::min<long long>((ncell + 255) / 256, std::min(c->part_cap, 2048));
   int cnt[2] = {0, 0};
@@ -786,8 +806,8 @@ int gpe_sim_linelength(gpe_ctx *c, double *length) {
     LinePoint *w = (LinePoint *)c->ll_buf, *sg = w + c->ll_cap;
     int *counts = (int *)(sg + c->ll_cap);
     HIPCHK(hipMemsetAsync(counts, 0, 2 * sizeof(int), c->stream));
-    hipError_t e = launch_linelength(c->prec, c->psi, c->comp ? c->lo : nullptr, c->P.nx, c->P.ny, c->P.nz, c->part,
-                                     nb, w, sg, counts, c->ll_cap, c->stream);
+    hipError_t e = launch_linelength(prec, psi, lo, c->P.nx, c->P.ny, c->P.nz, c->part, nb, w, sg, counts, c->ll_cap,
+                                     c->stream);
     if (e != hipSuccess) return fail(GPE_EHIP, "linelength launch: %s", hipGetErrorString(e));
     HIPCHK(hipMemcpyAsync(cnt, counts, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -806,8 +826,103 @@ int gpe_sim_linelength(gpe_ctx *c, double *length) {
   if (cnt[1]) HIPCHK(hipMemcpy(sg.data(), dw + c->ll_cap, sizeof(LinePoint) * cnt[1], hipMemcpyDeviceToHost));
   double sum = 0.0;
   for (double v : part) sum += v;
-  sum += c->prec == GPE_FP32 ? ll_singles<float>(w, sg) : ll_singles<double>(w, sg);
-  *length = c->prec == GPE_FP32 ? (double)((float)sum * (float)c->P.dx) : sum * c->P.dx;
+  sum += prec == GPE_FP32 ? ll_singles<float>(w, sg) : ll_singles<double>(w, sg);
+  *length = prec == GPE_FP32 ? (double)((float)sum * (float)c->P.dx) : sum * c->P.dx;
+  return GPE_OK;
+}
+
+// The unitary forward DFT of the resident field into c->spec.
+int spectral_of_field(gpe_ctx *c) {
+  if (!c->spec && hipMalloc(&c->spec, c->n * 16) != hipSuccess) {
+    c->spec = nullptr;
+    return fail(GPE_ENOMEM, "spectral work field of %zu bytes", c->n * 16);
+  }
+  if (!c->spec_aux &&
+      hipMalloc(&c->spec_aux, sizeof(double) * spectral_scratch_doubles(c->P.nx, c->P.ny, c->P.nz)) != hipSuccess) {
+    c->spec_aux = nullptr;
+    return fail(GPE_ENOMEM, "spectral scratch");
+  }
+  return spectral_forward(c->P.nx, c->P.ny, c->P.nz, c->prec, c->psi, c->comp ? c->lo : nullptr,
+                          static_cast<double2 *>(c->spec), c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpe_sim_condensed_particles(gpe_ctx *c, gpe_condensed *out) {
+  if (!c || !out) return fail(GPE_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  if (c->P.nx < 2) return fail(GPE_EINVAL, "condensed_particles needs nx >= 2");
+  double M = 0.0;
+  if (int rc = gpe_sim_mass(c, &M)) return rc;
+  if (int rc = spectral_of_field(c)) return rc;
+  double2 a0;
+  HIPCHK(hipMemcpyAsync(&a0, c->spec, sizeof a0, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const double m0 = std::hypot(a0.x, a0.y);
+  const double n0 = m0 * m0;
+  const double V = (double)c->n;
+  const double rho0 = n0 / V;
+  double tot = 0.0;
+  if (int rc = spectral_condensed_sum(c->P.nx, c->P.ny, c->P.nz, c->P.dx, c->P.dy, c->P.dz, rho0,
+                                      static_cast<double *>(c->spec_aux), &tot, c->stream))
+    return rc;
+  // io.f90:371-381, with xr = -xl (the grid runs from -xr to xr)
+  const double box = 8.0 * (-c->P.xl) * (-c->P.yl) * (-c->P.zl);
+  const double temp = (M - n0) / tot;
+  const double E0 = (1.0 / (2.0 * V)) * (M * M + (M - n0) * (M - n0));
+  const double H = E0 + temp * (V - 1.0);
+  out->time = c->st.t;
+  out->im_t = c->st.im_t;
+  out->density = M / box;
+  out->n0_per_site = n0 / V;
+  out->mass = M;
+  out->n0 = n0;
+  out->temp = temp;
+  out->temp2 = ((M / box) - (n0 / V)) / tot;
+  out->energy_per_site = H / V;
+  return GPE_OK;
+}
+
+int gpe_sim_spectrum(gpe_ctx *c, int32_t *nharm, double *eta, double *idf, int32_t nidf) {
+  if (!c || !nharm || !eta || (nidf > 0 && !idf)) return fail(GPE_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  const int nb = c->P.nx / 2;
+  if (nidf != nb) return fail(GPE_EINVAL, "idf holds nx/2 = %d values, not %d", nb, nidf);
+  if (int rc = spectral_of_field(c)) return rc;
+  long long nh[7];
+  double es[7];
+  std::vector<double> bins(nb);
+  double *aux = static_cast<double *>(c->spec_aux);
+  if (int rc = spectral_spectrum(c->P.nx, c->P.ny, c->P.nz, static_cast<const double2 *>(c->spec), nb, aux,
+                                 reinterpret_cast<long long *>(aux + (size_t)c->P.nz * (7 + nb)), nh, es, bins.data(),
+                                 c->stream))
+    return rc;
+  // io.f90:685-689: the shell mean, left at 0 where the shell holds nothing
+  for (int m = 0; m < 7; ++m) {
+    nharm[m] = (int32_t)nh[m];
+    eta[m] = es[m] == 0.0 ? 0.0 : es[m] / (double)nh[m];
+  }
+  double acc = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    acc += bins[b];
+    idf[b] = acc;
+  }
+  return GPE_OK;
+}
+
+int gpe_sim_filtered(gpe_ctx *c, double filter_kc, double m, void *out, int32_t host_type, double *length) {
+  if (!c || !out) return fail(GPE_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  if (host_type != GPE_HOST_C128 && host_type != GPE_HOST_C64) return fail(GPE_EINVAL, "host type");
+  if (!(filter_kc != 0.0)) return fail(GPE_EINVAL, "filter_kc must be non-zero");
+  if (int rc = spectral_of_field(c)) return rc;
+  double2 *a = static_cast<double2 *>(c->spec);
+  if (int rc = spectral_filter(c->P.nx, c->P.ny, c->P.nz, a, m, filter_kc, c->stream)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (int rc = field_download(GPE_FP64, (size_t)c->P.nx * c->P.ny, c->P.nz, a, nullptr, out, host_type)) return rc;
+  if (length) return linelength_of(c, GPE_FP64, a, nullptr, length);
   return GPE_OK;
 }
 

@@ -100,7 +100,7 @@ int plan_for(int nx, int ny, int nz, hipfftHandle *out) {
   hipfftHandle plan = 0;
   // (nz, ny, nx) with x fastest: hipFFT's first size is the slowest
   const hipfftResult fr = hipfftPlan3d(&plan, nz, ny, nx, HIPFFT_Z2Z);
-  if (fr != HIPFFT_SUCCESS) return fail(GPE_EHIP, "random phase: hipfftPlan3d: %s", fft_err(fr));
+  if (fr != HIPFFT_SUCCESS) return fail(GPE_EHIP, "hipfftPlan3d: %s", fft_err(fr));
   g_plans[key] = plan;
   *out = plan;
   return GPE_OK;
@@ -230,6 +230,18 @@ done:
   if (d_part) hipFree(d_part);
   if (own && f) hipFree(f);
   return rc;
+}
+
+int fft_z2z(int nx, int ny, int nz, void *f, int direction, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  hipfftHandle plan = 0;
+  if (int rc = plan_for(nx, ny, nz, &plan)) return rc;
+  hipfftResult fr = hipfftSetStream(plan, s);
+  if (fr == HIPFFT_SUCCESS)
+    fr = hipfftExecZ2Z(plan, (hipfftDoubleComplex *)f, (hipfftDoubleComplex *)f,
+                       direction < 0 ? HIPFFT_FORWARD : HIPFFT_BACKWARD);
+  if (fr != HIPFFT_SUCCESS) return fail(GPE_EHIP, "3-D FFT: %s", fft_err(fr));
+  return GPE_OK;
 }
 
 }  // namespace gpe

@@ -17,4 +17,8 @@ int random_phase_modes(int nx, int ny, int nz, double dx, double dy, double dz, 
 // lo for GPE_FP32C), on stream s; returns when it is written.
 int random_phase_device(int nx, int ny, int nz, double dx, double dy, double dz, double kmax, uint64_t seed,
                         int prec, void *out, void *lo, hipStream_t s);
+// In-place unnormalised 3-D DFT of a complex128 (nx, ny, nz) field, x
+// fastest, with the cached plan of the grid: direction -1 forward
+// (exp(-i k.x)), +1 backward; queued on stream s.
+int fft_z2z(int nx, int ny, int nz, void *f, int direction, hipStream_t s);
 }  // namespace gpe

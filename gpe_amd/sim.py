@@ -208,6 +208,39 @@ class Simulation:
         p = self.state()["p"]
         return [write_zeros(proc_dir, w, p, self.zeros(w), self.cfg.nz) for w in ("zeros", "re_zeros", "im_zeros")]
 
+    # -- FFT-side diagnostics (unitary forward DFT of psi on the device) ---
+    def condensed_particles(self):
+        """condensed_particles (io.f90:304-395): the mass.dat record as a
+        dict (time, im_t, density, n0_per_site, mass, n0, temp, temp2,
+        energy_per_site)."""
+        r = _lib.GpeCondensed()
+        _lib.check(self._L.gpe_sim_condensed_particles(self._ctx, ctypes.byref(r)))
+        return {n: getattr(r, n) for n, _ in r._fields_}
+
+    def spectrum(self):
+        """spectrum (io.f90:623-716): (nharm[7], eta[7], idf[nx/2]) - modes
+        and mean occupation per octave shell, and the integral distribution
+        F(kk), kk = 1..nx/2."""
+        nb = self.cfg.nx // 2
+        nh = np.zeros(7, np.int32)
+        eta = np.zeros(7)
+        idf = np.zeros(max(nb, 1))
+        _lib.check(self._L.gpe_sim_spectrum(self._ctx, nh.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                            eta.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                            idf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), nb))
+        return nh, eta, idf[:nb]
+
+    def filtered(self, filter_kc, m=1.0, with_length=False, single=False):
+        """filtered_surface (io.f90:524-619): the field low-pass filtered by
+        max(1 - m k^2/filter_kc^2, 0) in k-space, as an (nz, ny, nx) array
+        (complex64 if single), and with_length its vortex-line length."""
+        out = np.empty(self.cfg.shape, np.complex64 if single else np.complex128)
+        ln = ctypes.c_double()
+        _lib.check(self._L.gpe_sim_filtered(self._ctx, float(filter_kc), float(m), _ptr(out),
+                                            _lib.GPE_HOST_C64 if single else _lib.GPE_HOST_C128,
+                                            ctypes.byref(ln) if with_length else None))
+        return (out, ln.value) if with_length else out
+
     def renormalise(self, nn, norm):
         _lib.check(self._L.gpe_sim_renormalise(self._ctx, float(nn), float(norm)))
 

@@ -74,22 +74,68 @@ def write_dens(proc_dir, psi, cfg, p, t=0.0, im_t=0.0):
     int32 js, je, ks, ke; complex(pr) psi(0:nx1, js:je, ks:ke); real(pr)
     x(0:nx1), y(0:ny1), z(0:nz1).  Appends p to p_saved.dat ('(i10)') in the
     parent directory, as rank 0 does.  Returns the file path."""
+    os.makedirs(proc_dir, exist_ok=True)
+    path = os.path.join(proc_dir, "dens" + itos(p) + ".dat")
+    _write_dump(path, t + im_t, psi, cfg)
+    with open(os.path.join(os.path.dirname(os.path.abspath(proc_dir.rstrip("/"))), "p_saved.dat"), "a") as fh:
+        fh.write("%10d\n" % p)
+    return path
+
+
+def _write_dump(path, tval, psi, cfg):
     psi = np.ascontiguousarray(psi)
     rt = np.float64 if psi.dtype == np.complex128 else np.float32
     nz, ny, nx = psi.shape
     x, y, z = (np.asarray(a, dtype=rt) for a in cfg.coords())
-    os.makedirs(proc_dir, exist_ok=True)
-    path = os.path.join(proc_dir, "dens" + itos(p) + ".dat")
     with open(path, "wb") as fh:
-        fh.write(np.array([t + im_t], dtype=rt).tobytes())
+        fh.write(np.array([tval], dtype=rt).tobytes())
         fh.write(np.array([nx, ny, nz, 1, 1, 0, ny - 1, 0, nz - 1], dtype=np.int32).tobytes())
         fh.write(psi.tobytes())
         fh.write(x.tobytes())
         fh.write(y.tobytes())
         fh.write(z.tobytes())
-    with open(os.path.join(os.path.dirname(os.path.abspath(proc_dir.rstrip("/"))), "p_saved.dat"), "a") as fh:
-        fh.write("%10d\n" % p)
+
+
+def write_filtered(proc_dir, psi, cfg, p, t=0.0):
+    """filtered<p>.dat of filtered_surface(a, 0) (io.f90:586-596): the dens
+    layout (read_dens reads it) with t in place of t+im_t.  Returns the
+    path.  end_state_filtered.dat (flag 1) is the end_state layout:
+    write_state."""
+    os.makedirs(proc_dir, exist_ok=True)
+    path = os.path.join(proc_dir, "filtered" + itos(p) + ".dat")
+    _write_dump(path, t, psi, cfg)
     return path
+
+
+MASS_FIELDS = ("time", "im_t", "density", "n0_per_site", "mass", "n0", "temp", "temp2", "energy_per_site")
+
+
+def append_mass(path, rec):
+    """One condensed_particles record appended to mass.dat, '(9e17.9)'
+    (io.f90:376-381); rec is Simulation.condensed_particles()'s dict."""
+    with open(path, "a") as fh:
+        fh.write("".join(fortran_e(rec[k]) for k in MASS_FIELDS) + "\n")
+
+
+def write_spectrum(root, p, t, nharm, eta, idf):
+    """The files of spectrum (io.f90:691-708) under root: spectrum/
+    spectrum<p>.dat ('(2i9,e17.9)' per shell), idf/idf<p>.dat ('(i9,e17.9)'
+    per kk) and a '(4e17.9)' line t, eta(1:3) appended to eta_time.dat.
+    Returns the three paths."""
+    for d in ("spectrum", "idf"):
+        os.makedirs(os.path.join(root, d), exist_ok=True)
+    ps = os.path.join(root, "spectrum", "spectrum" + itos(p) + ".dat")
+    with open(ps, "w") as fh:
+        for m in range(len(eta)):
+            fh.write("%9d%9d%s\n" % (m + 1, int(nharm[m]), fortran_e(eta[m])))
+    pi = os.path.join(root, "idf", "idf" + itos(p) + ".dat")
+    with open(pi, "w") as fh:
+        for kk, v in enumerate(idf, start=1):
+            fh.write("%9d%s\n" % (kk, fortran_e(v)))
+    pe = os.path.join(root, "eta_time.dat")
+    with open(pe, "a") as fh:
+        fh.write(fortran_e(t) + "".join(fortran_e(v) for v in eta[:3]) + "\n")
+    return ps, pi, pe
 
 
 def read_dens(path, single=False):

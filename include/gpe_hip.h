@@ -196,6 +196,41 @@ int gpe_random_phase_prepare(int32_t nx, int32_t ny, int32_t nz);
  * (re, im pairs) and sets *count. */
 int gpe_random_phase_modes(int32_t nx, int32_t ny, int32_t nz, double dx, double dy, double dz, double kmax,
                            uint64_t seed, int64_t *flat_index, double *phasor, int64_t cap, int64_t *count);
+/* FFT-side diagnostics of the resident field.  Each starts from the
+ * unitary forward DFT a of psi, as the reference's fft(.., 'forward')
+ * leaves it (ic.f90:910-939: 2DECOMP&FFT's forward transform divided by
+ * sqrt(nx*ny*nz)), computed here with hipFFT in FP64 whatever the storage
+ * form; a(i,j,k) takes wavenumber index i = 0..nx-1 (likewise j, k), with
+ * k^2 formed by the reference's wrap (i <= (nx-1)/2+1 ? i : nx-i)^2 + ...
+ * They need one complex128 work field beside the context's (allocated on
+ * first use).
+ *
+ * condensed_particles (io.f90:304-395): the record it appends to mass.dat,
+ * from M = mass(psi) and n0 = |a(0,0,0)|^2, with the grid extent
+ * xr = -xl (likewise y, z) and time = the context's t. */
+typedef struct gpe_condensed {
+  double time, im_t;
+  double density;          /* M / (8 xr yr zr) */
+  double n0_per_site;      /* n0 / V, V = nx*ny*nz */
+  double mass, n0;         /* M, n0 */
+  double temp, temp2;      /* (M-n0)/tot, (M/(8 xr yr zr) - n0/V)/tot */
+  double energy_per_site;  /* H / V */
+} gpe_condensed;
+int gpe_sim_condensed_particles(gpe_ctx *ctx, gpe_condensed *out);
+/* spectrum (io.f90:623-716): for the 7 octave shells
+ * m-1 <= |log2(|k|/(2 pi))| < m, the number of modes nharm[m-1] and the
+ * mean occupation eta[m-1] (0 for an empty shell), and the integral
+ * distribution idf[kk-1] = sum of |a|^2 over modes with 0 < |k| <= kk,
+ * kk = 1..nx/2 (nidf must be nx/2).  Written by the reference to
+ * spectrum<p>.dat, idf<p>.dat and eta_time.dat. */
+int gpe_sim_spectrum(gpe_ctx *ctx, int32_t *nharm, double *eta, double *idf, int32_t nidf);
+/* filtered_surface (io.f90:524-619): the field whose spectrum is a times
+ * max(1 - m k^2/filter_kc^2, 0) (m = fscale*ind, or 1), transformed back
+ * with the unitary inverse DFT, into the host array out (nx*ny*nz of
+ * host_type); with length != NULL also its vortex-line length (the
+ * reference's save_linelength(filtered, 1)).  The resident field is
+ * unchanged. */
+int gpe_sim_filtered(gpe_ctx *ctx, double filter_kc, double m, void *out, int32_t host_type, double *length);
 /* renormalise (variables.f90:715-726): psi = sqrt(nn)*psi/sqrt(norm). */
 int gpe_sim_renormalise(gpe_ctx *ctx, double nn, double norm);
 /* f(psi) of the resident field into a host buffer (get_rhs, solve.f90:382). */

@@ -41,6 +41,7 @@ def test_struct_layouts_match_header():
     # gpe_line_param: 7 doubles, 2 int32; gpe_zero_point: 2 int32, int64, 3 doubles
     assert ctypes.sizeof(_lib.GpeLineParam) == 7 * 8 + 2 * 4
     assert _lib.ZERO_DTYPE.itemsize == 2 * 4 + 8 + 3 * 8
+    assert ctypes.sizeof(_lib.GpeCondensed) == 9 * 8
 
 
 def test_invalid_params_rejected_without_gpu_work():
@@ -67,6 +68,10 @@ def test_new_entry_points_reject_bad_arguments_without_gpu_work():
     assert L.gpe_sim_zeros(None, 0, None, 0, ctypes.byref(n)) == -1
     assert L.gpe_sim_random_phase(None, ctypes.c_uint64(1), 0.5) == -1
     assert L.gpe_sim_imprint_vortex_line(None, ctypes.byref(_lib.GpeLineParam()), 1.0) == -1
+    assert L.gpe_sim_condensed_particles(None, ctypes.byref(_lib.GpeCondensed())) == -1
+    assert L.gpe_sim_spectrum(None, None, None, None, 0) == -1
+    assert L.gpe_sim_filtered(None, 1.0, 1.0, None, 0, None) == -1
+    assert L.gpe_tune_load(None, None, None) == -1
     assert L.gpe_transfer_bytes(None, ctypes.byref(i64()), ctypes.byref(i64())) == -1
     st = _lib.GpeStepState()
     assert L.gpe_solver_step_ex(None, None, 2, None, ctypes.byref(st), 3) == -1
