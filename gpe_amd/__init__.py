@@ -2,12 +2,14 @@
 
 Modules:
   sim        Simulation: one field resident on the GPU (solver steps, norm,
-             energy, line length, vortex cores, imprinting, file dumps)
+             energy, line length, vortex cores, condensate and spectrum,
+             filtered field, imprinting, file dumps)
   batch      Batch: many uneven simulations advanced together
   serve      Server: continuous batching of a request stream
   config     RunConfig: the reference's run parameters
   ic         host-side initial states
-  state_io   the reference's end_state, dens and zeros file formats
+  state_io   the reference's end_state, dens, zeros, mass, spectrum and
+             filtered file formats
   tuning     launch-configuration table (tile variant, z-chunk per stage)
   accuracy   compensated FP32 against FP64 (psi, energy, line length, cores)
   workloads  the batch and serving workloads

@@ -921,7 +921,16 @@ int gpe_sim_filtered(gpe_ctx *c, double filter_kc, double m, void *out, int32_t 
   double2 *a = static_cast<double2 *>(c->spec);
   if (int rc = spectral_filter(c->P.nx, c->P.ny, c->P.nz, a, m, filter_kc, c->stream)) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (int rc = field_download(GPE_FP64, (size_t)c->P.nx * c->P.ny, c->P.nz, a, nullptr, out, host_type)) return rc;
+  if (host_type == GPE_HOST_C128) {
+    if (int rc = field_download(GPE_FP64, (size_t)c->P.nx * c->P.ny, c->P.nz, a, nullptr, out, host_type)) return rc;
+  } else {
+    // the filtered field is FP64: round it to complex64 on the host
+    std::vector<double> tmp(2 * c->n);
+    if (int rc = field_download(GPE_FP64, (size_t)c->P.nx * c->P.ny, c->P.nz, a, nullptr, tmp.data(), GPE_HOST_C128))
+      return rc;
+    float *o = static_cast<float *>(out);
+    for (size_t q = 0; q < 2 * c->n; ++q) o[q] = (float)tmp[q];
+  }
   if (length) return linelength_of(c, GPE_FP64, a, nullptr, length);
   return GPE_OK;
 }

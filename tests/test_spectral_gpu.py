@@ -46,6 +46,7 @@ def test_spectral_diagnostics_match_oracle(shape, precision):
         nh, eta, idf = sim.spectrum()
         f, ln = sim.filtered(0.6 * cfg.nx / 2, m=1.0, with_length=True)
         f2 = sim.filtered(3.0, m=2.0)
+        f32 = sim.filtered(3.0, m=2.0, single=True)
         st = sim.state()
         after = sim.download().astype(np.complex128)
     assert np.array_equal(after, held)  # the resident field is untouched
@@ -64,6 +65,7 @@ def test_spectral_diagnostics_match_oracle(shape, precision):
     scale = np.max(np.abs(ff))
     assert np.max(np.abs(f - ff)) < 1e-13 * scale
     assert np.max(np.abs(f2 - S.filtered(a, 3.0, 2.0))) < 1e-13 * scale
+    assert f32.dtype == np.complex64 and np.array_equal(f32, f2.astype(np.complex64))
     assert ln == pytest.approx(pyoracle.linelength(cfg, ff), rel=1e-10)
 
 

@@ -86,3 +86,34 @@ def test_batch_checkpoint_restart_bitwise(tmp_path):
     for (w, qw), (g, qg) in zip(want, got):
         assert np.array_equal(w, g)
         assert qw["t"] == qg["t"] and qw["dt"] == qg["dt"]
+
+
+def test_filtered_file_is_the_dens_layout_with_t(tmp_path):
+    c = _cfg()
+    psi = np.arange(c.nx * c.ny * c.nz, dtype=np.float64).reshape(c.shape) * (1 - 0.5j)
+    path = state_io.write_filtered(str(tmp_path / "proc0000"), psi, c, p=7, t=2.5)
+    assert os.path.basename(path) == "filtered0000007.dat"
+    got = state_io.read_dens(path)
+    assert np.array_equal(got[0], psi) and got[1] == 2.5  # t, not t + im_t (io.f90:588)
+    assert not (tmp_path / "p_saved.dat").exists()
+
+
+def test_mass_and_spectrum_records(tmp_path):
+    rec = dict(zip(state_io.MASS_FIELDS, [1.5, 0.0, 0.98, 2.8e-5, 1.2e6, 3.7e3, 0.64, -1.0e-3, 0.66]))
+    state_io.append_mass(str(tmp_path / "mass.dat"), rec)
+    state_io.append_mass(str(tmp_path / "mass.dat"), rec)
+    lines = open(tmp_path / "mass.dat").read().splitlines()
+    # '(9e17.9)': nine 17-character fields
+    assert len(lines) == 2 and len(lines[0]) == 9 * 17
+    assert [float(lines[0][17 * q:17 * q + 17]) for q in range(9)] == pytest.approx(list(rec.values()), rel=1e-8)
+    assert lines[0][:17] == "  0.150000000E+01"
+    ps, pi, pe = state_io.write_spectrum(str(tmp_path), 12, 0.5, [3, 10, 0, 0, 0, 0, 0],
+                                         [1.25, 0.5, 0, 0, 0, 0, 0], [1.0, 2.0, 4.0])
+    assert ps.endswith(os.path.join("spectrum", "spectrum0000012.dat"))
+    sl = open(ps).read().splitlines()
+    # '(2i9,e17.9)'
+    assert len(sl) == 7 and sl[0] == "        1        3  0.125000000E+01"
+    il = open(pi).read().splitlines()
+    assert il == ["        1  0.100000000E+01", "        2  0.200000000E+01", "        3  0.400000000E+01"]
+    # '(4e17.9)': t, eta(1:3)
+    assert open(pe).read() == "  0.500000000E+00  0.125000000E+01  0.500000000E+00  0.000000000E+00\n"
