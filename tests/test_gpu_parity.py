@@ -292,6 +292,30 @@ def test_line_pair_256cube(gpu, oracle):
     assert rel_l2(got, want) < 1e-12
 
 
+def test_random_phase_512_rk45_fp64_matches_oracle(gpu, oracle):
+    # the headline workload at full size (512^3 random phase, periodic,
+    # 4th order, RKF45 tol 1e-6): one accepted step of the GPU's FP64 path
+    # against the oracle on the same field, with the same dt and t
+    from gpe_amd.sim import Simulation
+    cfg = RunConfig(nx=512, ny=512, nz=512, xr=xr_for_spacing(0.5, 512), bcs=1, order=4, scheme="rk45",
+                    tau=0.05, g=1.0, mu=1.0)
+    say("512^3 random phase rk45 fp64 vs oracle")
+    with Simulation(cfg, None) as sim:
+        sim.random_phase(seed=20261202, kmax=0.5)
+        psi = sim.download()
+        sim.advance(1)
+        got = sim.download()
+        s = sim.state()
+    st = oracle.make_state(cfg.dt)
+    want = oracle.solver(cfg, st, psi, NT)
+    del psi
+    err = rel_l2(got, want)
+    say("rel L2 %.3e, t %.12f / %.12f, rejects %d / %d" % (err, s["t"], st.t, s["n_rejects"], st.n_rejects))
+    assert err < 1e-12
+    assert s["t"] == pytest.approx(st.t, rel=1e-12)
+    assert s["dt"].real == pytest.approx(st.dt_re, rel=1e-12)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32c"])
 def test_plane_wave_512_rk45_matches_scalar_recursion(gpu, precision):
     # full config-2 size: a plane wave stays a plane wave; every site must

@@ -82,3 +82,25 @@ def test_uniform_field_is_all_condensate():
     assert rec["n0"] == pytest.approx(V * abs(c) ** 2, rel=1e-13)
     assert rec["mass"] == pytest.approx(V * abs(c) ** 2 * dV, rel=1e-13)
     assert np.max(idf) < 1e-20 * V and np.max(eta) < 1e-20 * V
+
+
+@pytest.mark.parametrize("precision", ["fp32c", "fp64"])
+def test_parseval_and_identity_filter_at_256cube(precision):
+    # size-independent properties on a random-phase field whose modes all
+    # lie within |k| <= nx/2: F(nx/2) + n0 = sum |psi|^2 = M / dV, and a
+    # filter with an infinite cut-off returns the field
+    from gpe_amd.sim import Simulation
+    n = 256
+    cfg = _cfg((n, n, n), precision)
+    print("[case] spectral 256^3", precision, flush=True)
+    with Simulation(cfg, None) as sim:
+        sim.random_phase(seed=20261202, kmax=0.5)
+        rec = sim.condensed_particles()
+        nh, eta, idf = sim.spectrum()
+        held = sim.download()
+        f = sim.filtered(1e12)
+    dV = cfg.dx * cfg.dy * cfg.dz
+    assert idf[-1] + rec["n0"] == pytest.approx(rec["mass"] / dV, rel=1e-12)
+    assert np.all(np.diff(idf) >= 0)
+    assert nh.sum() <= n ** 3 - 1
+    assert np.max(np.abs(f - held)) < 1e-13 * np.max(np.abs(held))
