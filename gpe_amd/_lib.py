@@ -53,7 +53,7 @@ EXPORTS = [
     "gpe_transfer_bytes", "gpe_sim_imprint_vortex_line", "gpe_sim_random_phase", "gpe_random_phase_prepare", "gpe_random_phase_modes", "gpe_batch_add_random_phase",
     "gpe_sim_upload", "gpe_sim_download", "gpe_sim_set_state", "gpe_sim_get_state",
     "gpe_sim_advance", "gpe_sim_norm_history", "gpe_sim_norm", "gpe_sim_energy",
-    "gpe_sim_mass", "gpe_sim_linelength", "gpe_sim_zeros", "gpe_sim_condensed_particles", "gpe_sim_spectrum", "gpe_sim_filtered", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
+    "gpe_sim_mass", "gpe_sim_hamiltonian", "gpe_sim_linelength", "gpe_sim_zeros", "gpe_sim_condensed_particles", "gpe_sim_spectrum", "gpe_sim_filtered", "gpe_sim_renormalise", "gpe_sim_rhs", "gpe_prof_enable", "gpe_prof_query",
     "gpe_tune_set", "gpe_set_fusion", "gpe_tune_get", "gpe_tune_key", "gpe_tune_load", "gpe_autotune", "gpe_probe_bandwidth",
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_footprint", "gpe_batch_add",
     "gpe_batch_remove", "gpe_batch_run", "gpe_batch_sync", "gpe_batch_poll", "gpe_batch_query",
@@ -130,6 +130,7 @@ def lib():
         "gpe_sim_energy": ([vp, dp], ctypes.c_int),
         "gpe_sim_mass": ([vp, dp], ctypes.c_int),
         "gpe_sim_linelength": ([vp, dp], ctypes.c_int),
+        "gpe_sim_hamiltonian": ([vp, dp], ctypes.c_int),
         "gpe_sim_zeros": ([vp, i32, vp, i64, ctypes.POINTER(i64)], ctypes.c_int),
         "gpe_sim_condensed_particles": ([vp, ctypes.POINTER(GpeCondensed)], ctypes.c_int),
         "gpe_sim_spectrum": ([vp, ctypes.POINTER(i32), dp, dp, i32], ctypes.c_int),

@@ -771,6 +771,34 @@ double ll_singles(std::vector<LinePoint> &w, std::vector<LinePoint> &s) {
 
 extern "C" {
 
+int gpe_sim_hamiltonian(gpe_ctx *c, double *h) {
+  if (!c || !h) return fail(GPE_EINVAL, "NULL argument");
+  if (!c->uploaded) return fail(GPE_ESTATE, "no field uploaded");
+  if (c->P.eqn != 4) return fail(GPE_EINVAL, "the Hamiltonian is that of eqn_to_solve = 4");
+  if (c->part_cap < PW_BLOCKS) return fail(GPE_EINVAL, "internal: partial buffer below %d", PW_BLOCKS);
+  double E = 0.0;
+  if (int rc = gpe_sim_energy(c, &E)) return rc;
+  double *p2 = c->part, *p4 = c->part + c->part_cap, *pv = c->part + 2 * (size_t)c->part_cap;
+  hipError_t e = launch_moments(c->prec == GPE_FP64 ? 0 : 1, c->psi, c->comp ? c->lo : nullptr, c->base, p2, p4, pv,
+                                c->stream);
+  if (e != hipSuccess) return fail(GPE_EHIP, "moments launch: %s", hipGetErrorString(e));
+  std::vector<double> hp(3 * (size_t)c->part_cap);
+  HIPCHK(hipMemcpyAsync(hp.data(), c->part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double s2 = 0.0, s4 = 0.0, sv = 0.0;
+  for (int b = 0; b < PW_BLOCKS; ++b) {
+    s2 += hp[b];
+    s4 += hp[c->part_cap + b];
+    sv += hp[2 * (size_t)c->part_cap + b];
+  }
+  const double dv = c->P.dx * c->P.dy * c->P.dz;
+  // energy() = (sum Re(-lap psi psi*) + 1/2 |psi|^4) dV: twice the kinetic
+  // energy plus half the quartic sum
+  const double kin = 0.5 * (E - 0.5 * s4 * dv);
+  *h = kin + (0.5 * c->P.g * s4 + sv - c->P.mu * s2) * dv;
+  return GPE_OK;
+}
+
 }  // extern "C"
 
 namespace {

@@ -130,6 +130,20 @@ hipError_t launch_norm_f64(const void *psi, int nx, int ny, int nz, int weighted
   return hipGetLastError();
 }
 
+hipError_t launch_moments(int prec, const void *psi, const void *lo, const StageArgs &A, double *p2, double *p4,
+                          double *pv, hipStream_t s) {
+  if (prec == 0)
+    hipLaunchKernelGGL((moments_kernel<double, false>), dim3(PW_BLOCKS), dim3(256), 0, s, psi, nullptr, A.nx, A.ny,
+                       A.nz, A.xl, A.yl, A.zl, A.dx, A.dy, A.dz, A.ox2, A.oy2, A.oz2, p2, p4, pv);
+  else if (lo)
+    hipLaunchKernelGGL((moments_kernel<float, true>), dim3(PW_BLOCKS), dim3(256), 0, s, psi, lo, A.nx, A.ny, A.nz,
+                       A.xl, A.yl, A.zl, A.dx, A.dy, A.dz, A.ox2, A.oy2, A.oz2, p2, p4, pv);
+  else
+    hipLaunchKernelGGL((moments_kernel<float, false>), dim3(PW_BLOCKS), dim3(256), 0, s, psi, nullptr, A.nx, A.ny,
+                       A.nz, A.xl, A.yl, A.zl, A.dx, A.dy, A.dz, A.ox2, A.oy2, A.oz2, p2, p4, pv);
+  return hipGetLastError();
+}
+
 hipError_t launch_norm_f32(const void *psi, const void *lo, int nx, int ny, int nz, int weighted, double w1,
                            double w2, double w3, double *part, hipStream_t s) {
   if (lo)

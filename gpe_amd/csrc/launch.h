@@ -74,6 +74,9 @@ hipError_t launch_reduce(const double *pn, const double *pe, const double *pen, 
 constexpr int PW_BLOCKS = 1024;
 hipError_t launch_norm_f64(const void *psi, int nx, int ny, int nz, int weighted, double w1, double w2,
                            double w3, double *part, hipStream_t s);
+// Block partials (PW_BLOCKS each) of sum |psi|^2, |psi|^4, V |psi|^2 (prec 0: FP64 storage).
+hipError_t launch_moments(int prec, const void *psi, const void *lo, const StageArgs &A, double *p2, double *p4,
+                          double *pv, hipStream_t s);
 hipError_t launch_norm_f32(const void *psi, const void *lo, int nx, int ny, int nz, int weighted, double w1,
                            double w2, double w3, double *part, hipStream_t s);
 hipError_t launch_renorm_f64(void *psi, size_t n, double a, DivC b, hipStream_t s);

@@ -42,6 +42,53 @@ __global__ void __launch_bounds__(256) norm_kernel(const void *psi, const void *
   if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+// Sums of |psi|^2, |psi|^4 and V |psi|^2 (V the trap of ic.f90:713-731 at
+// x = xl + i dx, ...), in FP64 whatever the storage form: the terms of the
+// eqn-4 Hamiltonian besides the kinetic one.  Fixed-order block partials,
+// as norm_kernel.
+template <typename T, bool COMP>
+__global__ void __launch_bounds__(256)
+    moments_kernel(const void *psi, const void *lo, int nx, int ny, int nz, double xl, double yl, double zl,
+                   double dx, double dy, double dz, double ox2, double oy2, double oz2, double *p2, double *p4,
+                   double *pv) {
+  __shared__ double red[3][4];
+  const size_t n = (size_t)nx * ny * nz;
+  const size_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const size_t beg = (size_t)blockIdx.x * chunk;
+  const size_t end = beg + chunk < n ? beg + chunk : n;
+  double a2 = 0.0, a4 = 0.0, av = 0.0;
+  for (size_t s = beg + threadIdx.x; s < end; s += blockDim.x) {
+    cx<double> v = todbl(ldc<T>(psi, s));
+    if constexpr (COMP) v = v + todbl(ldc<T>(lo, s));
+    const double m2 = v.re * v.re + v.im * v.im;
+    const int i = (int)(s % nx);
+    const size_t r = s / nx;
+    const int j = (int)(r % ny);
+    const int k = (int)(r / ny);
+    const double x = xl + (double)i * dx, y = yl + (double)j * dy, z = zl + (double)k * dz;
+    const double V = 0.5 * ((ox2 * (x * x) + oy2 * (y * y)) + oz2 * (z * z));
+    a2 += m2;
+    a4 += m2 * m2;
+    av += V * m2;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a2 += __shfl_xor(a2, o);
+    a4 += __shfl_xor(a4, o);
+    av += __shfl_xor(av, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a2;
+    red[1][threadIdx.x >> 6] = a4;
+    red[2][threadIdx.x >> 6] = av;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    p2[blockIdx.x] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+    p4[blockIdx.x] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+    pv[blockIdx.x] = ((red[2][0] + red[2][1]) + red[2][2]) + red[2][3];
+  }
+}
+
 // renormalise (variables.f90:715-726): psi = (a*psi)/b, a = sqrt(nn),
 // b = sqrt(norm), with the reference's complex / real division.
 static __global__ void __launch_bounds__(256) renorm_kernel_f64(void *psi, size_t n, double a, DivC b) {

@@ -150,6 +150,11 @@ class Simulation:
     def mass(self):
         return self._scalar(self._L.gpe_sim_mass)
 
+    def hamiltonian(self):
+        """The eqn-4 Hamiltonian the dynamics conserve (kinetic energy of
+        the finite-difference Laplacian + trap + quartic - mu N)."""
+        return self._scalar(self._L.gpe_sim_hamiltonian)
+
     def save_end_state(self, path):
         """The field and step state in the reference's end_state format
         (io.f90:843-853)."""

@@ -141,6 +141,14 @@ int gpe_sim_norm_history(gpe_ctx *ctx, double *out, int64_t n);
 int gpe_sim_norm(gpe_ctx *ctx, double *norm);     /* get_norm */
 int gpe_sim_energy(gpe_ctx *ctx, double *energy); /* energy */
 int gpe_sim_mass(gpe_ctx *ctx, double *mass);     /* mass */
+/* The eqn-4 Hamiltonian that the dynamics conserve,
+ * H = K + sum (V |psi|^2 + g/2 |psi|^4 - mu |psi|^2) dV with the kinetic
+ * energy K = (1/2) sum Re(-lap psi psi*) dV of the finite-difference
+ * Laplacian, formed from energy() (variables.f90:580-604, which weighs
+ * 2K + (1/2)|psi|^4) and one pass for the other sums.  No reference
+ * counterpart: the reference reports energy() only.  eqn_to_solve must be
+ * 4 (GPE_EINVAL otherwise). */
+int gpe_sim_hamiltonian(gpe_ctx *ctx, double *hamiltonian);
 /* linelength (variables.f90:896-1162): total vortex-line length, the
  * reference's cell-by-cell zero tracing (including its reuse of the previous
  * cell's second point in cells that find a single point). */
