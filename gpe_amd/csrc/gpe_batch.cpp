@@ -187,8 +187,8 @@ int group_of(gpe_batch *b, int prec, int order, int scheme) {
 }
 
 void drop_graph(gpe_batch *b) {
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
 }
@@ -206,10 +206,10 @@ int grow_slots(gpe_batch *b, int want) {
       hipMalloc((void **)&nc, sizeof(InstCtl) * (size_t)ncap) != hipSuccess ||
       hipMalloc((void **)&nact, sizeof(int) * (size_t)ncap) != hipSuccess ||
       hipMalloc((void **)&nr, 2 * sizeof(int) * (size_t)ncap) != hipSuccess) {
-    if (na) hipFree(na);
-    if (nc) hipFree(nc);
-    if (nact) hipFree(nact);
-    if (nr) hipFree(nr);
+    if (na) (void)hipFree(na);
+    if (nc) (void)hipFree(nc);
+    if (nact) (void)hipFree(nact);
+    if (nr) (void)hipFree(nr);
     return fail(GPE_ENOMEM, "batch tables for %d instances", ncap);
   }
   GPE_HIPCHK(hipMemset(nc, 0, sizeof(InstCtl) * (size_t)ncap));
@@ -221,10 +221,10 @@ int grow_slots(gpe_batch *b, int want) {
                            hipMemcpyDeviceToDevice));
     GPE_HIPCHK(hipMemcpy(nc, b->d_ctl, sizeof(InstCtl) * b->cap, hipMemcpyDeviceToDevice));
     GPE_HIPCHK(hipMemcpy(nact, b->d_active, sizeof(int) * b->cap, hipMemcpyDeviceToDevice));
-    hipFree(b->d_args);
-    hipFree(b->d_ctl);
-    hipFree(b->d_active);
-    hipFree(b->d_range);
+    (void)hipFree(b->d_args);
+    (void)hipFree(b->d_ctl);
+    (void)hipFree(b->d_active);
+    (void)hipFree(b->d_range);
   }
   b->d_args = na;
   b->d_ctl = nc;
@@ -249,7 +249,7 @@ int take_block(gpe_batch *b, size_t bytes, void **out) {
   while (b->in_use + b->pooled + bytes > b->budget && !b->pool.empty()) {
     auto last = std::prev(b->pool.end());
     b->pooled -= last->first;
-    hipFree(last->second);
+    (void)hipFree(last->second);
     b->pool.erase(last);
   }
   if (b->in_use + bytes > b->budget)
@@ -378,7 +378,7 @@ void pair_tiles(const Group &G, int vid, int *TX, int *TY) {
 int upload_table(KindTable &T) {
   const int n = (int)T.h.size();
   if (n > T.cap) {
-    if (T.d) hipFree(T.d);
+    if (T.d) (void)hipFree(T.d);
     T.d = nullptr;
     int c = std::max(n, 2 * T.cap);
     if (hipMalloc((void **)&T.d, sizeof(WorkItem) * (size_t)c) != hipSuccess)
@@ -393,8 +393,8 @@ int upload_table(KindTable &T) {
 
 int ensure_partials(gpe_batch *b, int need) {
   if (need <= b->part_cap) return GPE_OK;
-  if (b->d_pn) hipFree(b->d_pn);
-  if (b->d_pe) hipFree(b->d_pe);
+  if (b->d_pn) (void)hipFree(b->d_pn);
+  if (b->d_pe) (void)hipFree(b->d_pe);
   b->d_pn = b->d_pe = nullptr;
   int c = std::max(need, 2 * b->part_cap);
   if (hipMalloc((void **)&b->d_pn, sizeof(double) * (size_t)c) != hipSuccess ||
@@ -496,7 +496,7 @@ int capture(gpe_batch *b) {
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   if (rc) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (e != hipSuccess) return fail(GPE_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
@@ -581,23 +581,23 @@ int gpe_batch_create(int64_t hbm_budget, gpe_batch **out) {
 
 void gpe_batch_destroy(gpe_batch *b) {
   if (!b) return;
-  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
   for (Inst &I : b->slot)
-    if (I.block) hipFree(I.block);
-  for (auto &kv : b->pool) hipFree(kv.second);
+    if (I.block) (void)hipFree(I.block);
+  for (auto &kv : b->pool) (void)hipFree(kv.second);
   for (Group &G : b->groups)
     for (KindTable &T : G.tab)
-      if (T.d) hipFree(T.d);
+      if (T.d) (void)hipFree(T.d);
   void *ptrs[] = {b->d_args, b->d_ctl, b->d_active, b->d_range, b->d_pn, b->d_pe};
   for (void *p : ptrs)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   for (hipStream_t q : b->aux)
-    if (q) hipStreamDestroy(q);
+    if (q) (void)hipStreamDestroy(q);
   for (hipEvent_t ev : b->ev_done)
-    if (ev) hipEventDestroy(ev);
-  if (b->ev_fork) hipEventDestroy(b->ev_fork);
-  if (b->stream) hipStreamDestroy(b->stream);
+    if (ev) (void)hipEventDestroy(ev);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
 
@@ -967,14 +967,14 @@ int gpe_batch_autotune(gpe_batch *b, int32_t reps) {
             break;
           }
           float ms = 0.f;
-          hipEventElapsedTime(&ms, e0, e1);
+          (void)hipEventElapsedTime(&ms, e0, e1);
           if (ms < best) {
             best = ms;
             bv = v;
             bz = z;
           }
         }
-      if (T.d) hipFree(T.d);
+      if (T.d) (void)hipFree(T.d);
       if (rc) break;
       G.vid[k] = bv;
       G.zc[k] = bz;
@@ -1017,14 +1017,14 @@ int gpe_batch_autotune(gpe_batch *b, int32_t reps) {
             break;
           }
           float ms = 0.f;
-          hipEventElapsedTime(&ms, e0, e1);
+          (void)hipEventElapsedTime(&ms, e0, e1);
           if (ms < best) {
             best = ms;
             bv = v;
             bz = z;
           }
         }
-      if (T.d) hipFree(T.d);
+      if (T.d) (void)hipFree(T.d);
       G.pvid[q] = bv;
       G.pzc[q] = bz;
       G.pfuse[q] = best < 0.97f * (kbest[k] + kbest[k + 1]);
@@ -1032,8 +1032,8 @@ int gpe_batch_autotune(gpe_batch *b, int32_t reps) {
     }
     if (rc) break;
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   b->dirty = true;
   return rc;
 }

@@ -205,7 +205,7 @@ hipEvent_t take_event(gpe_ctx *c) {
 void prof_drain(gpe_ctx *c) {
   for (auto &ev : c->pending) {
     float ms = 0.f;
-    hipEventSynchronize(ev.b);
+    (void)hipEventSynchronize(ev.b);
     if (hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess && ev.kind >= 0 && ev.kind < GPE_PROF_KINDS) {
       c->prof_n[ev.kind] += 1;
       c->prof_ms[ev.kind] += ms;
@@ -223,12 +223,12 @@ int run_kind(gpe_ctx *c, int kind, StageArgs &A) {
   if (c->prof) {
     ea = take_event(c);
     eb = take_event(c);
-    if (ea) hipEventRecord(ea, c->stream);
+    if (ea) (void)hipEventRecord(ea, c->stream);
   }
   hipError_t e = launch_stage((int)c->esz, c->vid[kind], kind, c->R, c->comp, A, c->stream);
   if (e != hipSuccess) return fail(GPE_EHIP, "stage %s launch: %s", kind_name(kind), hipGetErrorString(e));
   if (c->prof && ea && eb) {
-    hipEventRecord(eb, c->stream);
+    (void)hipEventRecord(eb, c->stream);
     c->pending.push_back({kind, ea, eb});
     if (c->pending.size() > 4096) prof_drain(c);
   }
@@ -242,12 +242,12 @@ int run_pair(gpe_ctx *c, int pair, const StageArgs &A, StageArgs &B) {
   if (c->prof) {
     ea = take_event(c);
     eb = take_event(c);
-    if (ea) hipEventRecord(ea, c->stream);
+    if (ea) (void)hipEventRecord(ea, c->stream);
   }
   hipError_t e = launch_pair((int)c->esz, c->pvid[pair], pair, c->R, c->comp, A, B, c->stream);
   if (e != hipSuccess) return fail(GPE_EHIP, "pair %s launch: %s", kind_name(KP0 + pair), hipGetErrorString(e));
   if (c->prof && ea && eb) {
-    hipEventRecord(eb, c->stream);
+    (void)hipEventRecord(eb, c->stream);
     c->pending.push_back({KP0 + pair, ea, eb});
     if (c->pending.size() > 4096) prof_drain(c);
   }
@@ -388,7 +388,8 @@ struct Ctl {
 
 template <typename RT>
 int rkqs(gpe_ctx *c, double *hist_slot) {
-  const RT safety = (RT)c->P.safety, eps = (RT)c->P.eps;
+  // eps divides the error on the device (the final stage reports maxval/eps)
+  const RT safety = (RT)c->P.safety;
   const RT dt_inc = (RT)c->P.dt_increase, dt_dec = (RT)c->P.dt_decrease;
   const RT errcon = Ctl<RT>::pw((RT)5.0 / safety, (RT)1.0 / dt_inc);
   RT dtr = (RT)c->st.dt_re, dti = (RT)c->st.dt_im;
@@ -492,14 +493,14 @@ void free_all(gpe_ctx *c) {
   void *ptrs[] = {c->psi, c->psi2, c->lo, c->lo2, c->k[0], c->k[1], c->k[2], c->k[3], c->k[4], c->k[5],
                   c->acc, c->scale, c->part, c->dscal, c->hist, c->ll_buf, c->zr_buf, c->spec, c->spec_aux};
   for (void *p : ptrs)
-    if (p) hipFree(p);
-  if (c->hscal) hipHostFree(c->hscal);
+    if (p) (void)hipFree(p);
+  if (c->hscal) (void)hipHostFree(c->hscal);
   for (auto &ev : c->pending) {
-    hipEventDestroy(ev.a);
-    hipEventDestroy(ev.b);
+    (void)hipEventDestroy(ev.a);
+    (void)hipEventDestroy(ev.b);
   }
-  for (auto e : c->pool) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
 int check_params(const gpe_params *p) {
@@ -626,7 +627,7 @@ bad:
 
 void gpe_ctx_destroy(gpe_ctx *ctx) {
   if (!ctx) return;
-  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_all(ctx);
   delete ctx;
 }
@@ -667,7 +668,7 @@ int gpe_sim_advance(gpe_ctx *c, int64_t nsteps, int32_t record_norm) {
   if (nsteps < 0) return fail(GPE_EINVAL, "nsteps < 0");
   if (record_norm) {
     if (c->hist_cap < nsteps) {
-      if (c->hist) hipFree(c->hist);
+      if (c->hist) (void)hipFree(c->hist);
       c->hist = nullptr;
       HIPCHK(hipMalloc((void **)&c->hist, sizeof(double) * (size_t)(nsteps > 0 ? nsteps : 1)));
       c->hist_cap = nsteps;
@@ -825,7 +826,7 @@ int linelength_of(gpe_ctx *c, int prec, const void *psi, const void *lo, double 
   for (;;) {
     if (!c->ll_buf || c->ll_cap < 1) {
       const int cap = c->ll_cap > 0 ? c->ll_cap : (1 << 16);
-      if (c->ll_buf) hipFree(c->ll_buf);
+      if (c->ll_buf) (void)hipFree(c->ll_buf);
       c->ll_buf = nullptr;
       if (hipMalloc(&c->ll_buf, 2 * sizeof(LinePoint) * (size_t)cap + 256) != hipSuccess)
         return fail(GPE_ENOMEM, "line-length lists of %d points", cap);
@@ -842,7 +843,7 @@ int linelength_of(gpe_ctx *c, int prec, const void *psi, const void *lo, double 
     const int need = std::max(cnt[0], cnt[1]);
     if (need <= c->ll_cap) break;
     // lists overflowed: grow them and run the pass again
-    hipFree(c->ll_buf);
+    (void)hipFree(c->ll_buf);
     c->ll_buf = nullptr;
     c->ll_cap = need + need / 4;
   }
@@ -1039,7 +1040,7 @@ int gpe_sim_zeros(gpe_ctx *c, int32_t which, gpe_zero_point *out, int64_t cap, i
     if (cnt <= c->zr_cap) break;
     // the list overflowed: grow it and run the pass again
     if (cnt > (1 << 28)) return fail(GPE_ENOMEM, "%d zero points", cnt);
-    hipFree(c->zr_buf);
+    (void)hipFree(c->zr_buf);
     c->zr_buf = nullptr;
     c->zr_cap = cnt + cnt / 4;
   }
@@ -1328,10 +1329,15 @@ int gpe_autotune(gpe_ctx *c, int32_t reps) {
         if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(GPE_EHIP, "event"); break; }
         for (int r = 0; r < reps && rc == GPE_OK; ++r) rc = run_pair(c, pair, A, B);
         if (rc) break;
-        hipEventRecord(e1, c->stream);
-        hipEventSynchronize(e1);
         float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
+        hipError_t he = hipEventRecord(e1, c->stream);
+        if (he == hipSuccess) he = hipEventSynchronize(e1);
+        if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+        if (he != hipSuccess) {
+          // a failed trial must not be taken for the fastest
+          rc = fail(GPE_EHIP, "autotune timing: %s", hipGetErrorString(he));
+          break;
+        }
         if (ms < best) {
           best = ms;
           bv = v;
@@ -1377,10 +1383,15 @@ int gpe_autotune(gpe_ctx *c, int32_t reps) {
         if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(GPE_EHIP, "event"); break; }
         for (int r = 0; r < reps && rc == GPE_OK; ++r) rc = run_kind(c, kind, A);
         if (rc) break;
-        hipEventRecord(e1, c->stream);
-        hipEventSynchronize(e1);
         float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
+        hipError_t he = hipEventRecord(e1, c->stream);
+        if (he == hipSuccess) he = hipEventSynchronize(e1);
+        if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+        if (he != hipSuccess) {
+          // a failed trial must not be taken for the fastest
+          rc = fail(GPE_EHIP, "autotune timing: %s", hipGetErrorString(he));
+          break;
+        }
         if (ms < best) {
           best = ms;
           bv = v;
@@ -1399,8 +1410,8 @@ int gpe_autotune(gpe_ctx *c, int32_t reps) {
     for (int pair : pairs) c->pfuse[pair] = pbest[pair] < 0.97f * (kbest[first[pair]] + kbest[first[pair] + 1]);
   }
   c->prof = prof;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return GPE_OK;

@@ -225,10 +225,10 @@ done:
   // on an early exit, let queued copies and kernels finish before their
   // buffers (host vectors included) go away
   if (rc) (void)hipStreamSynchronize(s);
-  if (d_idx) hipFree(d_idx);
-  if (d_v) hipFree(d_v);
-  if (d_part) hipFree(d_part);
-  if (own && f) hipFree(f);
+  if (d_idx) (void)hipFree(d_idx);
+  if (d_v) (void)hipFree(d_v);
+  if (d_part) (void)hipFree(d_part);
+  if (own && f) (void)hipFree(f);
   return rc;
 }
 
