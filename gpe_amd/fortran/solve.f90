@@ -15,8 +15,8 @@
 ! One rank only (nyprocs = nzprocs = 1): boundaries are applied in the GPU
 ! kernels, so the ghost planes of var_in are not read.
 !
-! Device-authoritative mode (environment GPE_RESIDENT=1, real time only, not
-! with diagnostic): the field stays in HBM across calls.  The unchanged
+! Device-authoritative mode (environment GPE_RESIDENT=1, not with
+! diagnostic): the field stays in HBM across calls.  The unchanged
 ! driver copies psi%new into psi%old and calls solver() every step
 ! (gpe.f90:167-176, 230), but it reads psi only for its periodic I/O
 ! (gpe.f90:178-228, 234, 245-247) and at the end of the run.  So the module
@@ -33,13 +33,23 @@
 ! those exact.  get_norm(psi%new) runs every step (gpe.f90:244); with
 ! `if (solver_norm(in_var, norm)) return` at its top it takes the norm of
 ! the device field instead of the stale host copy.
+!
+! Imaginary time (eqn 4) changes psi between calls: get_norm decides the
+! switch to real time and renormalise rescales psi (gpe.f90:243-273).  The
+! field stays resident there only once get_norm's hook has been called and,
+! when renorm is set, renormalise's too (`if (solver_renormalise(var,
+! norm)) return` at its top), and not with imprint_vl.  solver_norm then
+! repeats the driver's test abs((norm-prev_norm)/prev_norm) < 1e-12 with the
+! norms it returned and writes psi%new back in the iteration that switches
+! (idl_surface may read it there), or whenever the previous norm is not
+! its own.
 module solve
   use parameters
   use iso_c_binding
   implicit none
 
   private
-  public :: solver, solver_norm, solver_sync, solver_transfer_bytes
+  public :: solver, solver_norm, solver_sync, solver_renormalise, solver_transfer_bytes
 
   ! mirrors struct gpe_params (include/gpe_hip.h)
   type, bind(C) :: gpe_params_t
@@ -93,6 +103,12 @@ module solve
       integer(c_int64_t), intent(out) :: h2d, d2h
       integer(c_int) :: rc
     end function gpe_transfer_bytes
+    function gpe_sim_renormalise(ctx, nn, norm) bind(C, name='gpe_sim_renormalise') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: nn, norm
+      integer(c_int) :: rc
+    end function gpe_sim_renormalise
     function gpe_tune_load(ctx, path, applied) bind(C, name='gpe_tune_load') result(rc)
       import :: c_ptr, c_int, c_int32_t
       type(c_ptr), value :: ctx, path
@@ -119,6 +135,12 @@ module solve
   logical, save :: host_stale = .false. ! held_out is older than the device field
   type(c_ptr), save :: held_out = c_null_ptr
   integer, save :: n_io = 0, m_io = 0, ps_io = 0  ! mirrors of gpe.f90's n, m, ps
+  ! imaginary time: resident once get_norm's hook (and renormalise's, when
+  ! renorm is set) have been called, so both act on the device field
+  logical, save :: norm_hook = .false., renorm_hook = .false.
+  real (pr), save :: last_norm = 0.0_pr  ! the norm solver_norm last returned
+  logical, save :: have_last = .false.   ! ... in the iteration before this one
+  integer(c_int64_t), save :: gone_h2d = 0, gone_d2h = 0  ! bytes moved by destroyed contexts
 
   contains
 
@@ -135,12 +157,19 @@ module solve
     type(gpe_step_state_t) :: st
     integer(c_int) :: rc
     integer(c_int32_t) :: flags
+    integer(c_int64_t) :: h2d_now, d2h_now
     logical :: use_res
 
     ! gpe.f90 switches from imaginary to real time once the norm has
     ! converged (gpe.f90:250-264); the rk45 control of a context follows the
     ! real_time it was created with, so a switch makes a new one
     if (c_associated(ctx) .and. (real_time .neqv. ctx_real_time)) then
+      ! solver_norm writes the field back in the iteration that switches
+      if (dev_holds .and. host_stale) call emergency_stop('ERROR: the device-held field was not '// &
+        'written back before the switch to real time (get_norm must call solver_norm).')
+      call solver_transfer_bytes(h2d_now, d2h_now)
+      gone_h2d = h2d_now
+      gone_d2h = d2h_now
       call gpe_ctx_destroy(ctx)
       ctx = c_null_ptr
       dev_holds = .false.
@@ -148,7 +177,11 @@ module solve
     end if
     if (.not. c_associated(ctx)) call create_context()
 
-    use_res = resident .and. real_time .and. .not. diagnostic
+    ! imaginary time changes psi between steps through get_norm and
+    ! renormalise (gpe.f90:243-273): resident only with both hooked, and
+    ! not with imprint_vl
+    use_res = resident .and. .not. diagnostic .and. (real_time .or. (eqn_to_solve == 4 .and. norm_hook .and. &
+      (renorm_hook .or. .not. renorm) .and. .not. imprint_vl))
     flags = 0
     if (use_res) then
       if (.not. dev_holds) then
@@ -279,17 +312,87 @@ module solve
     real(c_double) :: v
     integer(c_int) :: rc
 
+    norm_hook = .true.
     solver_norm = .false.
     norm = 0.0_pr
-    if (.not. dev_holds) return
-    if (.not. c_associated(held_out, c_loc(in_var))) return
+    if (.not. dev_holds .or. .not. c_associated(held_out, c_loc(in_var))) then
+      have_last = .false.
+      return
+    end if
     rc = gpe_sim_norm(ctx, v)
     if (rc /= 0) call emergency_stop('ERROR: '//last_error())
     norm = real(v, pr)
     solver_norm = .true.
+    if (.not. real_time) then
+      ! gpe.f90:245-264 switches to real time (or stops) once
+      ! abs((norm-prev_norm)/prev_norm) < 1e-12, and then may read psi%new:
+      ! write it back in that iteration (and whenever prev_norm is unknown)
+      if (.not. have_last) then
+        call write_back(c_loc(in_var))
+      else if (abs((norm-last_norm)/last_norm) < 1e-12_pr) then
+        call write_back(c_loc(in_var))
+      end if
+    end if
+    last_norm = norm
+    have_last = .true.
 
     return
   end function solver_norm
+
+! ***************************************************************************
+
+  logical function solver_renormalise(var, norm)
+    ! renormalise (variables.f90:715-726) of the device field when var is
+    ! the field a device-authoritative step returned: .true. and the field
+    ! (and var, if it was current) renormalised.  .false. otherwise
+    ! (renormalise then works on var on the host).  Call it at the top of
+    ! renormalise:  if (solver_renormalise(var, norm)) return
+    use error
+    use parameters
+    implicit none
+
+    complex (pr), dimension(0:nx1,js:je,ks:ke), intent(inout), target :: var
+    real (pr), intent(in) :: norm
+    integer(c_int) :: rc
+    logical :: was_current
+
+    renorm_hook = .true.
+    solver_renormalise = .false.
+    if (.not. dev_holds) return
+    if (.not. c_associated(held_out, c_loc(var))) return
+    rc = gpe_sim_renormalise(ctx, real(nn, c_double), real(norm, c_double))
+    if (rc /= 0) call emergency_stop('ERROR: '//last_error())
+    was_current = .not. host_stale
+    host_stale = .true.
+    if (was_current) call solver_sync(var)
+    solver_renormalise = .true.
+
+    return
+  end function solver_renormalise
+
+! ***************************************************************************
+
+  subroutine write_back(addr)
+    ! Download the device field into the host array at addr if that is the
+    ! stale field (solver_sync for a caller that holds only the address)
+    use error
+    use parameters
+    implicit none
+
+    type(c_ptr), intent(in) :: addr
+    integer(c_int) :: rc
+    integer(c_int32_t) :: ht
+
+    if (.not. host_stale) return
+    if (.not. c_associated(held_out, addr)) return
+    ht = 0
+    if (pr /= kind(1.0d0)) ht = 1
+    rc = gpe_sim_download(ctx, addr, ht)
+    if (rc /= 0) call emergency_stop('ERROR: '//last_error())
+    host_stale = .false.
+
+    return
+  end subroutine write_back
 
 ! ***************************************************************************
 
@@ -302,8 +405,10 @@ module solve
 
     to_device = 0
     to_host = 0
-    if (.not. c_associated(ctx)) return
-    rc = gpe_transfer_bytes(ctx, to_device, to_host)
+    if (c_associated(ctx)) rc = gpe_transfer_bytes(ctx, to_device, to_host)
+    ! and what the contexts replaced at a switch to real time moved
+    to_device = to_device + gone_h2d
+    to_host = to_host + gone_d2h
 
     return
   end subroutine solver_transfer_bytes

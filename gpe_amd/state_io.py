@@ -39,19 +39,32 @@ def read_state(path, single=False):
     return psi, p, t, im_t, dt
 
 
-def write_run_in(path, cfg, end_time, io=None):
+def _nml_value(v):
+    if isinstance(v, bool):
+        return ".true." if v else ".false."
+    if isinstance(v, str):
+        return "'%s'" % v
+    return repr(int(v) if isinstance(v, (int, np.integer)) else float(v))
+
+
+def write_run_in(path, cfg, end_time, io=None, run_extra=None, eqn_extra=None):
     """run_params / eqn_params namelists (parameters.f90:40-56) for cfg,
     and io_params (parameters.f90:74-76) from the dict `io` (save_rate,
-    save_rate2, save_rate3, p_save) when given."""
+    save_rate2, save_rate3, p_save) when given.  run_extra / eqn_extra add
+    other entries of those namelists (e.g. renorm, nn)."""
     b = lambda v: ".true." if v else ".false."  # noqa: E731
     with open(path, "w") as fh:
         fh.write("&run_params\n")
+        for k, v in (run_extra or {}).items():
+            fh.write("  %s = %s\n" % (k, _nml_value(v)))
         fh.write("  tau = %r\n  end_time = %r\n" % (float(cfg.tau), float(end_time)))
         fh.write("  xr = %r\n  yr = %r\n  zr = %r\n" % (float(cfg.xr), float(cfg.yr), float(cfg.zr)))
         fh.write("  scheme = '%s'\n  eqn_to_solve = %d\n  bcs = %d\n  order = %d\n" %
                  (cfg.scheme, cfg.eqn, cfg.bcs, cfg.order))
         fh.write("  real_time = %s\n/\n" % b(cfg.real_time))
         fh.write("&eqn_params\n")
+        for k, v in (eqn_extra or {}).items():
+            fh.write("  %s = %s\n" % (k, _nml_value(v)))
         fh.write("  Urhs = %r\n  diss_amp = %r\n  g = %r\n  mu = %r\n" %
                  (float(cfg.urhs), float(cfg.diss_amp), float(cfg.g), float(cfg.mu)))
         fh.write("  omx = %r\n  omy = %r\n  omz = %r\n/\n" % (float(cfg.omx), float(cfg.omy), float(cfg.omz)))

@@ -22,7 +22,14 @@
 ! before solver() in iteration p; DRIVER_REAL_AT=p switches an
 ! imaginary-time run to real time at the top of iteration p the way
 ! gpe.f90:250-264 does once the norm has converged (real_time = .true.,
-! im_t = 0, dt = (|Im dt|, |Re dt|)).
+! im_t = 0, dt = (|Im dt|, |Re dt|)).  DRIVER_IMAG=1 runs the imaginary-
+! time part of the loop instead (eqn 4): get_norm before the loop and
+! renormalise if renorm is set (gpe.f90:112-113); each iteration the
+! relative norm change, the switch to real time once it is below 1e-12
+! (recording psi%new as idl_surface would, record 10) and renormalise
+! (gpe.f90:243-276), with get_norm and renormalise hooked as a maintainer
+! hooks variables.f90 (solver_norm, solver_renormalise; the host forms,
+! restated below, when the hooks decline).
 !
 ! Usage: dropin_driver <run.in> <state_in> <state_out>
 program dropin_driver
@@ -33,8 +40,8 @@ program dropin_driver
 
   complex (pr), allocatable :: old(:,:,:), new(:,:,:)
   character(len=512) :: fin_nml, fin, fout
-  integer :: u, io, nx_in, ny_in, nz_in, p_in, n, m, ps, stop_at, save_at, real_at, ios
-  real (pr) :: t_in, im_t_in, norm
+  integer :: u, io, nx_in, ny_in, nz_in, p_in, n, m, ps, stop_at, save_at, real_at, ios, imag, uim
+  real (pr) :: t_in, im_t_in, norm, prev_norm, relnorm
   complex (pr) :: dt_in
   logical :: run_exist, save_exist, hooked
   integer(c_int64_t) :: h2d, d2h
@@ -58,6 +65,7 @@ program dropin_driver
   stop_at = env_int('DRIVER_STOP_AT')
   save_at = env_int('DRIVER_SAVE_AT')
   real_at = env_int('DRIVER_REAL_AT')
+  imag = env_int('DRIVER_IMAG')
 
   myrank = 0
   js = 0; je = ny-1; ks = 0; ke = nz-1
@@ -80,13 +88,21 @@ program dropin_driver
 
   open (newunit=io, file='io.bin', access='stream', form='unformatted', status='replace')
   open (newunit=u, file='norms.txt', status='replace')
+  open (newunit=uim, file='imag.txt', status='replace')  ! p, real_time, norm, relnorm (DRIVER_IMAG)
 
   ps = int(t/p_save)
   n = int(t/save_rate2)
   m = int(t/save_rate3)
   old = 0.0_pr
+  prev_norm = 1.0_pr
+  relnorm = 0.0_pr
+  if (imag == 1) then
+    call get_norm(new, prev_norm)
+    if (renorm) call renormalise(new, prev_norm)
+  end if
   do while (t+im_t <= end_time)
     if (p == real_at .and. .not. real_time) then
+      call solver_sync(new)   ! a switch forced outside the norm test
       real_time = .true.
       im_t = 0.0_pr
       dt = cmplx(abs(aimag(dt)), abs(real(dt, pr)), pr)
@@ -131,14 +147,34 @@ program dropin_driver
       open (98, file='SAVE')
       close (98, status='delete')
     end if
-    hooked = solver_norm(new, norm)
-    if (hooked) write (u, '(i10,es26.17)') p, norm
+    if (imag == 1) then
+      call get_norm(new, norm)
+      relnorm = abs((norm-prev_norm)/prev_norm)
+      write (uim, '(i10,l2,2es26.17)') p, real_time, norm, relnorm
+    else
+      hooked = solver_norm(new, norm)
+      if (hooked) write (u, '(i10,es26.17)') p, norm
+    end if
     if (modulo(t+im_t, abs(dt)*save_rate) < abs(dt)) then
       call record(8, new)          ! get_norm(psi%new) for save_norm
     end if
+    if (imag == 1 .and. .not. real_time) then
+      if (relnorm < 1e-12_pr) then
+        real_time = .true.
+        im_t = 0.0_pr
+        ps = int(t/p_save)
+        n = int(t/save_rate2)
+        m = int(t/save_rate3)
+        call record(10, new)       ! idl_surface(psi%new) with save_3d
+        dt = cmplx(abs(aimag(dt)), abs(real(dt, pr)), pr)
+      end if
+      if (renorm) call renormalise(new, norm)
+    end if
+    prev_norm = norm
     p = p+1
   end do
   close (u)
+  close (uim)
   if (run_exist) then
     call solver_sync(new)
     call record(9, new)            ! end_state(psi%new, 1) after the loop
@@ -163,6 +199,53 @@ contains
     write (io) p, what, t
     write (io) f
   end subroutine record
+
+  subroutine get_norm(in_var, norm)
+    ! variables.f90:554-576 with its hook: the device norm when the drop-in
+    ! holds in_var, else the closed-rule integral on the host (weights
+    ! 3/8, 7/6, 23/24 at each end, variables.f90:49-51, 730-840)
+    complex (pr), intent(in) :: in_var(0:nx1,js:je,ks:ke)
+    real (pr), intent(out) :: norm
+    real (pr) :: w(0:max(nx1,ny1,nz1), 3), s, sy
+    integer :: i, j, k, a, nn_a(3)
+
+    if (solver_norm(in_var, norm)) then
+      write (u, '(i10,es26.17)') p, norm
+      return
+    end if
+    nn_a = (/ nx, ny, nz /)
+    w = 1.0_pr
+    do a = 1, 3
+      w(0, a) = 3.0_pr/8.0_pr
+      w(1, a) = 7.0_pr/6.0_pr
+      w(2, a) = 23.0_pr/24.0_pr
+      w(nn_a(a)-3, a) = 23.0_pr/24.0_pr
+      w(nn_a(a)-2, a) = 7.0_pr/6.0_pr
+      w(nn_a(a)-1, a) = 3.0_pr/8.0_pr
+    end do
+    norm = 0.0_pr
+    do k = ks, ke
+      sy = 0.0_pr
+      do j = js, je
+        s = 0.0_pr
+        do i = 0, nx1
+          s = s + w(i, 1)*abs(in_var(i,j,k))**2
+        end do
+        sy = sy + w(j, 2)*s*dx
+      end do
+      norm = norm + w(k, 3)*sy*dy
+    end do
+    norm = norm*dz
+  end subroutine get_norm
+
+  subroutine renormalise(var, norm)
+    ! variables.f90:715-726 with its hook
+    complex (pr), intent(inout) :: var(0:nx1,js:je,ks:ke)
+    real (pr), intent(in) :: norm
+
+    if (solver_renormalise(var, norm)) return
+    var = sqrt(nn) * var / sqrt(norm)
+  end subroutine renormalise
 
   integer function env_int(name)
     character(len=*), intent(in) :: name

@@ -108,10 +108,10 @@ def _records(path, cfg, single=False):
     return out
 
 
-def _run_mode(tmp_path, exe, cfg, psi, nsteps, io, resident, **env_extra):
+def _run_mode(tmp_path, exe, cfg, psi, nsteps, io, resident, run_extra=None, eqn_extra=None, **env_extra):
     d = tmp_path / ("resident" if resident else "plain")
     d.mkdir()
-    write_run_in(str(d / "run.in"), cfg, (nsteps - 0.5) * cfg.tau, io=io)
+    write_run_in(str(d / "run.in"), cfg, (nsteps - 0.5) * cfg.tau, io=io, run_extra=run_extra, eqn_extra=eqn_extra)
     write_state(str(d / "end_state0000.dat"), psi, dt=cfg.dt)
     open(d / "RUNNING", "w").close()
     env = dict(os.environ)
@@ -253,3 +253,49 @@ def test_dropin_other_equations(tmp_path, oracle, eqn):
     want, _ = oracle.run(cfg, oracle.make_state(cfg.dt), psi, n)
     assert p == n
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12
+
+
+@pytest.mark.gpu
+@need_bins
+@pytest.mark.parametrize("renorm", [True, False])
+def test_resident_imaginary_time_with_hooked_norm_and_renormalise(tmp_path, renorm):
+    """GPE_RESIDENT=1 in imaginary time, with get_norm and renormalise
+    hooked (solver_norm, solver_renormalise): the field stays on the GPU
+    through the imaginary phase, the norm-converged switch to real time
+    (gpe.f90:245-273) writes it back exactly where the switch reads it, and
+    records and end state match the plain drop-in, whose norm and
+    renormalise run on the host."""
+    cfg = cfg_for(scheme="rk4", bcs=1, real_time=False, tau=0.01, omx=0.0, omy=0.0, omz=0.0, g=1.0, mu=1.0)
+    # a uniform condensate at |psi|^2 = mu/g with a faint ripple: imaginary
+    # time damps the ripple, and the norm change falls below 1e-12
+    # (the ripple sets how many imaginary steps the norm change takes to
+    # fall below 1e-12: about ten here)
+    rng = np.random.default_rng(21)
+    psi = (1.0 + (4e-8 if renorm else 2e-9) * rng.standard_normal(cfg.shape)).astype(np.complex128)
+    n = 24
+    print("[case] resident imaginary time, renorm", renorm, flush=True)
+    extra = dict(run_extra={"renorm": renorm, "imprint_vl": False}, eqn_extra={"nn": 2.0 if renorm else 1.0})
+    IO_IM = {"save_rate": 4, "save_rate2": 1e9, "save_rate3": 1e9, "p_save": 1e9}
+    dp, h2p, d2p = _run_mode(tmp_path, BIN_D, cfg, psi, n, IO_IM, False, DRIVER_IMAG=1, **extra)
+    dr, h2r, d2r = _run_mode(tmp_path, BIN_D, cfg, psi, n, IO_IM, True, DRIVER_IMAG=1, **extra)
+    rp, rr = _records(str(dp / "io.bin"), cfg), _records(str(dr / "io.bin"), cfg)
+    # the switch happened, at the same iteration, in both modes
+    sw_p = [q for q, w, _, _ in rp if w == 10]
+    sw_r = [q for q, w, _, _ in rr if w == 10]
+    assert len(sw_p) == 1 and sw_p == sw_r and 2 <= sw_p[0] < n - 2
+    assert [(q, w) for q, w, _, _ in rp] == [(q, w) for q, w, _, _ in rr]
+    for (q, w, t, fp), (_, _, t2, fr) in zip(rp, rr):
+        assert t == t2
+        # host and device norms differ in summation order, so renormalised
+        # fields agree to rounding, not bitwise
+        assert np.max(np.abs(fp - fr)) <= 1e-13 * np.max(np.abs(fp)), (q, w)
+    got_p = read_state(str(dp / "out.dat"))
+    got_r = read_state(str(dr / "out.dat"))
+    assert got_p[1:] == got_r[1:]  # p, t, im_t, dt
+    assert np.max(np.abs(got_p[0] - got_r[0])) <= 1e-13 * np.max(np.abs(got_p[0]))
+    # the resident run uploaded once per context (imaginary, then real time)
+    # and downloaded only where the driver read psi
+    fb = cfg.nx * cfg.ny * cfg.nz * 16
+    assert h2p == got_p[1] * fb  # one upload per iteration (im_t restarts at the switch)
+    assert h2r == 2 * fb
+    assert d2r < d2p / 2
