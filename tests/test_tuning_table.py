@@ -64,6 +64,23 @@ def test_load_applies_entry_and_keeps_results(tmp_path):
         assert np.array_equal(sim.download(), want)
 
 
+def test_load_skips_other_json_values(tmp_path):
+    # other keys may hold any JSON value (escaped strings, nested objects,
+    # exponents, true/false/null): the reader skips them to find the entry
+    from gpe_amd.sim import Simulation
+    cfg = _cfg()
+    path = tmp_path / "mixed.json"
+    path.write_text('{"note": "a \\"quoted\\" {not, an} [entry]", "meta": {"a": [1, 2.5e-3, -4E+2, true, false, '
+                    'null], "b": {"c": []}, "d": {}}, "%s": {"rk4_1": [4, 8], "rk4_2": [2, 4]}, "tail": [[], {}]}'
+                    % tuning.key(cfg))
+    json.loads(path.read_text())  # the file is valid JSON
+    print("[case] mixed table", flush=True)
+    with Simulation(cfg, ic.vortex_ring(cfg, 1.5), tuned=False) as sim:
+        assert sim.tune_load(str(path))
+        got = sim.tune_get()
+        assert got["rk4_1"][:2] == (4, 8) and got["rk4_2"][:2] == (2, 4)
+
+
 def test_load_missing_and_malformed(tmp_path):
     from gpe_amd.sim import Simulation
     cfg = _cfg()
