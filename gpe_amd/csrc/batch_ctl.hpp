@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(64) batch_control(InstCtl *__restrict__ ctl, S
     if (em <= 1.0) {
       double nr, ni;
       if (em > C.errcon) {
-        const double pw = pow(em, C.dt_inc);
+        const double pw = refm::pow_cr(em, C.dt_inc);
         nr = (C.safety * C.dt_re) * pw;
         ni = (C.safety * C.dt_im) * pw;
       } else {
@@ -71,9 +71,9 @@ __global__ void __launch_bounds__(64) batch_control(InstCtl *__restrict__ ctl, S
         active[i] = 0;
         return;
       }
-      const double pw = pow(em, C.dt_dec);
+      const double pw = refm::pow_cr(em, C.dt_dec);
       const double ar = (C.safety * C.dt_re) * pw, ai = (C.safety * C.dt_im) * pw;
-      const double a = hypot(ar, ai), b = 0.1 * hypot(C.dt_re, C.dt_im);
+      const double a = refm::hypot_ref(ar, ai), b = 0.1 * refm::hypot_ref(C.dt_re, C.dt_im);
       const double m = a > b ? a : b;
       if (C.real_time) {
         C.dt_re = m;
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(64) batch_control(InstCtl *__restrict__ ctl, S
     A.hi = 0.5 * C.dt_im;
     A.sdtr = C.sdt_re;
     A.sdti = C.sdt_im;
-    A.inv_absdt = 1.0 / hypot(C.dt_re, C.dt_im);
+    A.inv_absdt = 1.0 / refm::hypot_ref(C.dt_re, C.dt_im);
     A.write_scale = C.first_attempt;
     A.psi = C.buf[C.cur];
     A.psi_out = C.buf[C.cur ^ 1];

@@ -33,6 +33,8 @@ using namespace gpe;
 namespace {
 
 constexpr int kDefaultVid = 0;
+// the default tile variant of stage kind k in a group of precision prec
+int default_vid(int prec, int k) { return variant_allowed(prec == GPE_FP64, k, kDefaultVid) ? kDefaultVid : kVidF64Ck; }
 constexpr int kDefaultZc = 32;
 constexpr int kMaxGrid = 2048;  // persistent workgroups per batched launch
 
@@ -174,7 +176,7 @@ int group_of(gpe_batch *b, int prec, int order, int scheme) {
   G.R = order / 2;
   scheme_kinds(scheme, &G.k0, &G.nk);
   for (int k = 0; k < K_COUNT; ++k) {
-    G.vid[k] = kDefaultVid;
+    G.vid[k] = default_vid(prec, k);
     G.zc[k] = kDefaultZc;
   }
   for (int q = 0; q < P_COUNT; ++q) {
@@ -891,6 +893,8 @@ int gpe_batch_tune_set(gpe_batch *b, int32_t precision, int32_t order, int32_t s
     return GPE_OK;
   }
   if (kind < G.k0 || kind >= G.k0 + G.nk) return fail(GPE_EINVAL, "kind %d is not a stage of scheme %d", kind, scheme);
+  if (!variant_allowed(precision == GPE_FP64, kind, variant))
+    return fail(GPE_EINVAL, "FP64 Cash-Karp stages do not run tile variant %d", variant);
   G.vid[kind] = variant;
   G.zc[kind] = zchunk;
   b->dirty = true;
@@ -913,7 +917,7 @@ int gpe_batch_tune_get(gpe_batch *b, int32_t precision, int32_t order, int32_t s
       if (zchunk) *zchunk = G.zc[kind];
       return GPE_OK;
     }
-  if (variant) *variant = kind >= 16 ? -1 : kDefaultVid;
+  if (variant) *variant = kind >= 16 ? -1 : default_vid(precision, kind);
   if (zchunk) *zchunk = kind >= 16 ? 0 : kDefaultZc;
   return GPE_OK;
 }
@@ -943,6 +947,7 @@ int gpe_batch_autotune(gpe_batch *b, int32_t reps) {
       KindTable T;
       for (int v = 0; v < N_TILE_VARIANTS; ++v)
         for (int z : zcs) {
+          if (!variant_allowed(G.prec == GPE_FP64, k, v)) continue;
           int TX, TY;
           kind_tiles(v, &TX, &TY);
           make_table(b, G, TX, TY, z, T.h, nullptr, 0);

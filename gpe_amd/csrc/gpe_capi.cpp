@@ -578,7 +578,7 @@ int gpe_ctx_create(const gpe_params *params, gpe_ctx **out) {
     // default launch configuration; partial buffers sized for the largest
     // grid any variant and z-chunk can produce
     for (int kk = 0; kk < K_COUNT; ++kk) {
-      c->vid[kk] = 1;
+      c->vid[kk] = variant_allowed(c->prec == GPE_FP64, kk, 1) ? 1 : kVidF64Ck;
       c->zc[kk] = 0;
     }
     // every pair runs as its two stages until the autotuner (or
@@ -780,22 +780,35 @@ int gpe_sim_hamiltonian(gpe_ctx *c, double *h) {
   double E = 0.0;
   if (int rc = gpe_sim_energy(c, &E)) return rc;
   double *p2 = c->part, *p4 = c->part + c->part_cap, *pv = c->part + 2 * (size_t)c->part_cap;
-  hipError_t e = launch_moments(c->prec == GPE_FP64 ? 0 : 1, c->psi, c->comp ? c->lo : nullptr, c->base, p2, p4, pv,
-                                c->stream);
-  if (e != hipSuccess) return fail(GPE_EHIP, "moments launch: %s", hipGetErrorString(e));
-  std::vector<double> hp(3 * (size_t)c->part_cap);
-  HIPCHK(hipMemcpyAsync(hp.data(), c->part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  // sums of |psi|^2, |psi|^4 and V |psi|^2 over the field (hi + lo in the
+  // compensated form)
+  auto moments = [&](const void *lo, double &s2, double &s4, double &sv) -> int {
+    hipError_t e = launch_moments(c->prec == GPE_FP64 ? 0 : 1, c->psi, lo, c->base, p2, p4, pv, c->stream);
+    if (e != hipSuccess) return fail(GPE_EHIP, "moments launch: %s", hipGetErrorString(e));
+    std::vector<double> hp(3 * (size_t)c->part_cap);
+    HIPCHK(hipMemcpyAsync(hp.data(), c->part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    s2 = s4 = sv = 0.0;
+    for (int b = 0; b < PW_BLOCKS; ++b) {
+      s2 += hp[b];
+      s4 += hp[c->part_cap + b];
+      sv += hp[2 * (size_t)c->part_cap + b];
+    }
+    return GPE_OK;
+  };
   double s2 = 0.0, s4 = 0.0, sv = 0.0;
-  for (int b = 0; b < PW_BLOCKS; ++b) {
-    s2 += hp[b];
-    s4 += hp[c->part_cap + b];
-    sv += hp[2 * (size_t)c->part_cap + b];
+  if (int rc = moments(c->comp ? c->lo : nullptr, s2, s4, sv)) return rc;
+  // energy() = (sum Re(-lap psi psi*) + 1/2 |psi|^4) dV: twice the kinetic
+  // energy plus half the quartic sum.  The compensated form's energy() reads
+  // the high word only, so the quartic sum taken out of it is the high
+  // word's too (the potential terms below keep hi + lo).
+  double s4e = s4;
+  if (c->comp) {
+    double a = 0.0, b = 0.0;
+    if (int rc = moments(nullptr, a, s4e, b)) return rc;
   }
   const double dv = c->P.dx * c->P.dy * c->P.dz;
-  // energy() = (sum Re(-lap psi psi*) + 1/2 |psi|^4) dV: twice the kinetic
-  // energy plus half the quartic sum
-  const double kin = 0.5 * (E - 0.5 * s4 * dv);
+  const double kin = 0.5 * (E - 0.5 * s4e * dv);
   *h = kin + (0.5 * c->P.g * s4 + sv - c->P.mu * s2) * dv;
   return GPE_OK;
 }
@@ -1161,6 +1174,8 @@ int gpe_tune_set(gpe_ctx *c, int32_t kind, int32_t variant, int32_t zchunk) {
   }
   if (kind < 0 || kind >= K_COUNT) return fail(GPE_EINVAL, "kind out of range");
   if (variant < 0 || variant >= N_TILE_VARIANTS) return fail(GPE_EINVAL, "variant out of range");
+  if (!variant_allowed(c->prec == GPE_FP64, kind, variant))
+    return fail(GPE_EINVAL, "FP64 Cash-Karp stages do not run tile variant %d", variant);
   if (zchunk < 0) return fail(GPE_EINVAL, "zchunk < 0");
   c->vid[kind] = variant;
   c->zc[kind] = zchunk;
@@ -1375,6 +1390,7 @@ int gpe_autotune(gpe_ctx *c, int32_t reps) {
     float best = 1e30f;
     int bv = c->vid[kind], bz = c->zc[kind];
     for (int v = 0; v < N_TILE_VARIANTS && rc == GPE_OK; ++v) {
+      if (!variant_allowed(c->prec == GPE_FP64, kind, v)) continue;
       for (int zc : zcs) {
         if (zc > c->P.nz) continue;
         c->vid[kind] = v;

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "gpe_ic.h"
+#include "refmath.h"
 #include "gpe_spectral.h"
 
 namespace gpe {
@@ -59,9 +60,9 @@ __global__ void div_real(double2 *a, size_t n, double d) {
   }
 }
 
-// abs(a)**2 with abs the complex modulus
+// abs(a)**2 with abs the complex modulus (the host libm's hypot, restated)
 __device__ inline double mod2(double2 v) {
-  const double m = hypot(v.x, v.y);
+  const double m = gpe::refm::hypot_ref(v.x, v.y);
   return m * m;
 }
 

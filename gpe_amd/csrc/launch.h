@@ -16,6 +16,20 @@ struct Tile {
 constexpr int N_TILE_VARIANTS = 5;
 constexpr Tile kTiles[N_TILE_VARIANTS] = {{4, 4}, {8, 2}, {16, 1}, {4, 2}, {8, 1}};
 
+// FP64 Cash-Karp stages do not run the (4,4) and (8,2) tiles.  An earlier
+// build gave wrong 4th-order FP64 RKF45 fields with exactly these two
+// shapes (every other shape, scheme and precision matched the oracle), and
+// no fault in the source was found.  They are the two shapes whose FP64
+// 4th-order last-stage instantiation is allocated the full 256 VGPRs and
+// still spills to scratch (484 and 292 B per lane in the single-simulation
+// kernel, 624 and 372 B batched); (16,1) spills too, but under a 128-VGPR
+// budget, and passes.  The FP64 Cash-Karp default is (8,1), which needs no
+// scratch in any stage.
+constexpr int kVidF64Ck = 4;
+constexpr bool variant_allowed(bool fp64, int kind, int vid) {
+  return vid >= 0 && vid < N_TILE_VARIANTS && !(fp64 && kind >= K_CK_1 && kind <= K_CK_6 && vid <= 1);
+}
+
 // Launch stage kernel `kind` (radius 1 or 2) over A.nblocks workgroups with
 // tile variant `vid`.  comp selects the compensated FP32 form.
 hipError_t launch_stage(int prec_bytes, int vid, int kind, int radius, bool comp, const StageArgs &A,

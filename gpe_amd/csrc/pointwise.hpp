@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(256) imprint_kernel(void *psi, void *lo, LineA
       stc<float>(psi, s, mk(m * vr, m * vi));
     } else {
       const cx<double> v = todbl(ldc<float>(psi, s)) + todbl(ldc<float>(lo, s));
-      const double m = hypot(v.re, v.im);
+      const double m = refm::hypot_ref(v.re, v.im);
       const cx<double> w = mk(m * vr, m * vi);
       const cx<float> hi = mk((float)w.re, (float)w.im);
       stc<float>(psi, s, hi);

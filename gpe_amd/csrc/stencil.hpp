@@ -33,6 +33,7 @@
 
 #include <type_traits>
 
+#include "refmath.h"
 #include "stage_args.h"
 
 namespace gpe {
@@ -130,8 +131,10 @@ __device__ __forceinline__ cx<float> cdiv(cx<float> a, cx<float> b) {
   return mk(ldexpf((ac + bd) / den, -e), ldexpf((bc - ad) / den, -e));
 }
 
-// abs() of a complex: hypot, as flang emits it (no underflow of |a|^2)
-__device__ __forceinline__ double cabs_(cx<double> a) { return hypot(a.re, a.im); }
+// abs() of a complex: hypot, as flang emits it (no underflow of |a|^2); in
+// FP64 the host libm's hypot restated (refmath.h), so |.| rounds as in the
+// reference's binary
+__device__ __forceinline__ double cabs_(cx<double> a) { return refm::hypot_ref(a.re, a.im); }
 __device__ __forceinline__ float cabs_(cx<float> a) { return hypotf(a.re, a.im); }
 
 // Index map for a ghost index i in [-R, n-1+R].

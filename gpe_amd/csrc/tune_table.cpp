@@ -123,6 +123,11 @@ struct Cursor {
       if (!string(it.name) || !expect(':') || !expect('[') || !integer(v) || !expect(',') || !integer(z) ||
           !expect(']'))
         return false;
+      // values that do not fit an int32 are invalid, not wrapped
+      if (v < INT32_MIN || v > INT32_MAX || z < INT32_MIN || z > INT32_MAX) {
+        err = "value out of range in " + it.name;
+        return false;
+      }
       it.variant = static_cast<int>(v);
       it.zchunk = static_cast<int>(z);
       out.push_back(it);

@@ -90,6 +90,7 @@ class Server:
         queue = deque()
         futures = {}
         running = {}      # batch id -> (request, admit time, sites)
+        resident_sites = 0
         if resume_dir is not None:
             with open(os.path.join(resume_dir, "requests.json")) as fh:
                 meta = json.load(fh)
@@ -99,8 +100,8 @@ class Server:
                 m = meta[str(old)]
                 r = Request(m["rid"], m["arrival"], self.batch.cfgs[new], m["steps"], m["ic_kind"], m["seed"])
                 running[new] = (r, clock(), r.cfg.nx * r.cfg.ny * r.cfg.nz)
+                resident_sites += r.cfg.nx * r.cfg.ny * r.cfg.nz
         done = []         # (request, arrival, admitted, finished, status, steps)
-        resident_sites = 0
         chunk = 4
         busy_s = 0.0
         su_done = 0.0
@@ -176,6 +177,7 @@ class Server:
                 psi = self.batch.download(iid) if self.deliver is not None else None
                 self.batch.remove(iid)
                 resident_sites -= sites
+                assert resident_sites >= 0, resident_sites
                 su_done += float(sites) * q["steps"]
                 done.append((r, r.arrival, tadm, tnow, status, q["steps"]))
                 if self.deliver is not None:

@@ -56,6 +56,7 @@ def test_hamiltonian_matches_oracle_sums(precision):
         psi = sim.download()
     k2, s4, sv, s2 = _parts(cfg, psi)
     want = 0.5 * k2 + 0.5 * cfg.g * s4 + sv - cfg.mu * s2
+    print("[result] hamiltonian %s rel err %.3e" % (precision, abs(H - want) / abs(want)), flush=True)
     # FP32C forms the Laplacian in single precision, FP32 also sums energy() in it
     assert H == pytest.approx(want, rel={"fp64": 1e-12, "fp32c": 1e-5, "fp32": 1e-4}[precision])
 

@@ -100,6 +100,21 @@ def test_load_missing_and_malformed(tmp_path):
             sim.tune_load(str(tmp_path / "range.json"))
         assert "rk4_4" in str(e.value)
         assert sim.tune_get() == before
+        # a value beyond int32 is rejected, not wrapped (4294967297 would wrap to 1)
+        for bad in ([4294967297, 0], [1, -4294967296]):
+            json.dump({tuning.key(cfg): dict(ODD, rk4_3=bad)}, open(tmp_path / "wrap.json", "w"))
+            with pytest.raises(_lib.GpeError) as e:
+                sim.tune_load(str(tmp_path / "wrap.json"))
+            assert e.value.code == _lib.GPE_EIO and "rk4_3" in str(e.value)
+            assert sim.tune_get() == before
+        # FP64 Cash-Karp stages refuse the (4,4) and (8,2) tiles
+        c45 = cfg.with_(scheme="rk45")
+        with Simulation(c45, ic.vortex_ring(c45, 1.5), tuned=False) as s45:
+            for v in (0, 1):
+                with pytest.raises(_lib.GpeError):
+                    s45.tune_set("ck_6", v, 0)
+            assert s45.tune_get()["ck_6"][0] == 4
+            s45.tune_set("ck_6", 3, 0)
 
 
 @pytest.mark.skipif(not os.path.exists(BIN_D), reason="drop-in harness not built (make -C oracle ref)")

@@ -1,0 +1,132 @@
+"""The batching and serving workloads at their own sizes.
+
+Batching: 64 independent simulations with seeded uneven cubic grids
+N in {32, 48, 64, 96, 128}, dx = 0.4, 2nd-order FD, reflective walls, RK4
+dt = 0.01 for 500 steps, complex64 storage with the compensated update, each
+in its own trap with a centred vortex ring (gpe_amd.workloads.batch_mix).
+Every compensated instance is held to 1e-5 relative L2 against the same
+instance run in FP64, and the three smallest FP64 instances to 1e-12 against
+the oracle (the reference's RK4, solve.f90:109-154, run 500 times).
+
+Serving: the seeded Poisson stream of 40 requests/s (N in {48..256}, both
+BCs, both orders, 200-5000 steps, RK4 or RKF45, FP32C or FP64, ring /
+line-pair / random-phase states) for a short real-clock window, drained.
+Every delivered result must equal the same request run alone, bitwise, and
+the FP64 requests with N <= 64 must be within 1e-12 of the oracle.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+NT = min(16, os.cpu_count() or 1)
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def _run_batch(insts, precision, states):
+    from gpe_amd.batch import Batch
+    with Batch() as b:
+        ids = [b.add(i.cfg.with_(precision=precision), s, i.steps) for i, s in zip(insts, states)]
+        assert all(i is not None for i in ids)
+        t0 = time.perf_counter()
+        done = b.run_to_completion(chunk=100)
+        el = time.perf_counter() - t0
+        assert all(done[i] == 1 for i in ids), done
+        out = [b.download(i) for i in ids]
+        q = [b.query(i) for i in ids]
+    return out, q, el
+
+
+def test_batch_mix_64_instances_500_steps(oracle):
+    from gpe_amd.workloads import batch_mix
+    insts = batch_mix()
+    assert len(insts) == 64 and all(i.steps == 500 for i in insts)
+    sizes = sorted({i.cfg.nx for i in insts})
+    say("batch mix: 64 instances, sizes", sizes, "500 rk4 steps")
+    states = [i.initial_state() for i in insts]
+    g32, q32, t32 = _run_batch(insts, "fp32c", states)
+    say("fp32c batch %.2f s" % t32)
+    g64, q64, t64 = _run_batch(insts, "fp64", states)
+    say("fp64 batch %.2f s" % t64)
+    errs = [rel_l2(a, b) for a, b in zip(g32, g64)]
+    say("fp32c vs fp64: max rel L2 %.3e" % max(errs))
+    assert max(errs) < 1e-5
+    for a, b in zip(q32, q64):
+        assert a["steps"] == b["steps"] == 500
+        assert a["t"] == pytest.approx(b["t"], rel=1e-12)
+    # the three smallest FP64 instances against the oracle
+    small = sorted(range(len(insts)), key=lambda k: (insts[k].cfg.nx, k))[:3]
+    for k in small:
+        cfg = insts[k].cfg.with_(precision="fp64")
+        st = oracle.make_state(cfg.dt)
+        want, wn = oracle.run(cfg, st, states[k], 500, NT, with_norms=True)
+        e = rel_l2(g64[k], want)
+        say("instance %d (N=%d) fp64 vs oracle rel L2 %.3e" % (k, cfg.nx, e))
+        assert e < 1e-12
+        assert abs(q64[k]["norm"] - wn[-1]) < 1e-12 * wn[-1]
+        assert q64[k]["t"] == pytest.approx(st.t, rel=1e-13)
+
+
+def _device_initial_state(r):
+    """The initial field of a request exactly as the server builds it."""
+    from gpe_amd.sim import Simulation
+    if r.ic_kind != "random_phase":
+        return r.initial_state()
+    with Simulation(r.cfg.with_(precision="fp64")) as sim:
+        sim.random_phase(r.seed, kmax=0.5)
+        return sim.download(np.complex128)
+
+
+def test_serve_stream_window_equal_alone_and_oracle(oracle):
+    from gpe_amd.batch import Batch
+    from gpe_amd.serve import Server
+    from gpe_amd.workloads import request_stream
+    window = 0.45
+    reqs = [r for r in request_stream(40.0, 120.0, 20261202) if r.arrival < window]
+    say("serve: %d requests arriving in the first %.2f s, real clock, drained" % (len(reqs), window))
+    got = {}
+
+    def deliver(r, psi, info):
+        got[r.rid] = (psi, info)
+
+    with Server(max_sites=64e6, deliver=deliver) as s:
+        m = s.serve(reqs, duration=window, drain=True)
+    say("served: completed %d ok %d in %.1f s, p50 %.2f s p99 %.2f s" %
+        (m["completed"], m["ok"], m["elapsed_s"], m["latency_p50_s"], m["latency_p99_s"]))
+    assert m["completed"] == len(reqs) and m["ok"] == len(reqs), m
+    for r in reqs:
+        with Batch() as b:
+            i = r.add_to(b)
+            b.run_to_completion(chunk=64)
+            want = b.download(i)
+            q = b.query(i)
+        g, info = got[r.rid]
+        assert info["steps"] == r.steps == q["steps"]
+        assert np.array_equal(g, want), r
+        assert info["t"] == q["t"] and info["dt"] == q["dt"]
+    checked = 0
+    for r in reqs:
+        if r.cfg.precision != "fp64" or r.cfg.nx > 64:
+            continue
+        t0 = time.perf_counter()
+        st = oracle.make_state(r.cfg.dt)
+        want, _ = oracle.run(r.cfg, st, _device_initial_state(r), r.steps, NT)
+        g, info = got[r.rid]
+        e = rel_l2(g, want)
+        say("request %d (N=%d %s %s, %d steps) vs oracle rel L2 %.3e, t %.9g vs %.9g (%.0f s)" %
+            (r.rid, r.cfg.nx, r.cfg.scheme, r.ic_kind, r.steps, e, info["t"], st.t, time.perf_counter() - t0))
+        assert e < 1e-12
+        assert info["t"] == pytest.approx(st.t, rel=1e-13)
+        checked += 1
+    assert checked >= 2
