@@ -48,6 +48,7 @@
  *     guard, solve.f90:180-212).
  */
 #include <math.h>
+#include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
 #include "gpe_oracle.h"
@@ -66,6 +67,37 @@ static inline REAL r_sqrt(REAL v) { return IS_FLOAT ? (REAL)sqrtf((float)v) : (R
 static inline REAL r_tanh(REAL v) { return IS_FLOAT ? (REAL)tanhf((float)v) : (REAL)tanh((double)v); }
 static inline REAL r_pow(REAL a, REAL b) { return IS_FLOAT ? (REAL)powf((float)a, (float)b) : (REAL)pow((double)a, (double)b); }
 static inline REAL r_abs(REAL v) { return v < 0 ? -v : v; }
+
+/* Step-control pow.  The reference calls the C library's pow (accurate to
+ * about 0.52 ulp, not correctly rounded) for errmax**dt_decrease and
+ * errmax**dt_increase (solve.f90:197-220); that is the default here.  After
+ * PFX_set_control_pow(1) the double build rounds those two powers correctly
+ * instead: libquadmath's powq (113-bit significand) rounded to double, with
+ * a counter of results lying so close to a double rounding boundary that
+ * the conversion might not be the correctly rounded value (expected never).
+ * A step control on the device, which cannot call the host library (the
+ * batched instances), rounds pow correctly; this mode pins it bit for bit.
+ * errcon (solve.f90:173) keeps the C library's pow in both modes. */
+static int g_ctl_pow_cr = 0;
+static long g_ctl_pow_ambiguous = 0;
+static double pow_cr_quad(double x, double y) {
+  __float128 r = powq((__float128)x, (__float128)y);
+  double d = (double)r;
+  if (isfinite(d) && d != 0.0) {
+    __float128 up = ((__float128)d + (__float128)nextafter(d, INFINITY)) / 2;
+    __float128 dn = ((__float128)d + (__float128)nextafter(d, -INFINITY)) / 2;
+    __float128 dist = fminq(fabsq(r - up), fabsq(r - dn));
+    if (dist <= fabsq(r) * 0x1p-100Q) g_ctl_pow_ambiguous++;
+  }
+  return d;
+}
+static inline REAL ctl_pow(REAL a, REAL b) {
+  if (!IS_FLOAT && g_ctl_pow_cr) return (REAL)pow_cr_quad((double)a, (double)b);
+  return r_pow(a, b);
+}
+void FN(set_control_pow)(int correctly_rounded) { g_ctl_pow_cr = correctly_rounded; }
+double FN(control_pow)(double x, double y) { return (double)ctl_pow((REAL)x, (REAL)y); }
+long FN(control_pow_ambiguous)(void) { return g_ctl_pow_ambiguous; }
 
 typedef struct { REAL re, im; } cx;
 
@@ -495,7 +527,7 @@ static int rkqs(const ctx *c, tstate *ts, const REAL *old, REAL *new_, work *w, 
     if (!(errmax == errmax)) return -1;
     if (errmax <= (REAL)1.0) break;
     if (++rejects > 1000) return -1;
-    REAL pw = r_pow(errmax, dt_dec);
+    REAL pw = ctl_pow(errmax, dt_dec);
     cx dtt = cx_make(safety * dt.re, safety * dt.im);
     dtt = cx_make(dtt.re * pw, dtt.im * pw); /* safety*dt*(errmax**dt_decrease) */
     REAL adt = cx_absv(dt);
@@ -506,7 +538,7 @@ static int rkqs(const ctx *c, tstate *ts, const REAL *old, REAL *new_, work *w, 
   }
   cx dt_next;
   if (errmax > errcon) {
-    REAL pw = r_pow(errmax, dt_inc);
+    REAL pw = ctl_pow(errmax, dt_inc);
     cx a = cx_make(safety * dt.re, safety * dt.im);
     dt_next = cx_make(a.re * pw, a.im * pw);
   } else {

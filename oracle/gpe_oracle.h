@@ -75,7 +75,10 @@ typedef struct orc_line {
   long P##_zeros(const orc_params *p, const REAL *f, int which,             \
                  orc_zero *out, long cap);                                   \
   int P##_imprint_vortex_line(const orc_params *p, REAL *f,                 \
-                              const orc_line *vl, double scal);
+                              const orc_line *vl, double scal);             \
+  void P##_set_control_pow(int correctly_rounded);                          \
+  double P##_control_pow(double x, double y);                               \
+  long P##_control_pow_ambiguous(void);
 
 ORC_DECL(orc64, double)
 ORC_DECL(orc32, float)

@@ -8,6 +8,7 @@ Fields are numpy arrays of shape (nz, ny, nx), C-contiguous, so x is fastest
 as in the reference's psi(0:nx1, js:je, ks:ke).  complex128 selects the FP64
 restatement (orc64_*), complex64 the FP32 one (orc32_*).
 """
+import contextlib
 import ctypes
 import os
 
@@ -68,7 +69,35 @@ def lib():
             f = getattr(_LIB, pfx + "_zeros"); f.argtypes = [P, vp, I, vp, ctypes.c_long]; f.restype = ctypes.c_long
             f = getattr(_LIB, pfx + "_imprint_vortex_line"); f.argtypes = [P, vp, ctypes.POINTER(OrcLine), D]
             f.restype = I
+            f = getattr(_LIB, pfx + "_set_control_pow"); f.argtypes = [I]; f.restype = None
+            f = getattr(_LIB, pfx + "_control_pow"); f.argtypes = [D, D]; f.restype = D
+            f = getattr(_LIB, pfx + "_control_pow_ambiguous"); f.argtypes = []; f.restype = ctypes.c_long
     return _LIB
+
+
+@contextlib.contextmanager
+def correctly_rounded_control_pow():
+    """Within the block the FP64 oracle's rk45 step control rounds
+    errmax**dt_decrease and errmax**dt_increase correctly (quad precision)
+    instead of calling the C library's pow as the reference does: the
+    rounding of a device step control (batched instances)."""
+    L = lib()
+    L.orc64_set_control_pow(1)
+    try:
+        yield
+    finally:
+        L.orc64_set_control_pow(0)
+
+
+def control_pow(x, y):
+    """The FP64 oracle's step-control pow in its current mode."""
+    return lib().orc64_control_pow(float(x), float(y))
+
+
+def control_pow_ambiguous():
+    """Correctly rounded control pows whose quad result lay within 2^-100
+    of a double rounding boundary (then the rounding is not certain)."""
+    return lib().orc64_control_pow_ambiguous()
 
 
 def params(cfg, shape, nthreads=1):

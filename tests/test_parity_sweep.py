@@ -3,10 +3,13 @@
 Every case draws a grid shape (ragged in x against the 64-wide tiles, odd
 and even y/z), order, boundary condition, scheme, tile variant and z-chunk
 length, runs three solver() calls on the GPU and on the oracle, and asks
-for 1e-12 relative L2 in psi and equal t and dt.  FP64 results do not
-depend on the launch configuration, so any case outside the bar means an
-indexing, chunking or synchronisation fault in one configuration.  The draws
-are fixed by the seed; the sweep is ordinary regression coverage."""
+for the oracle's field, t and dt bit for bit (the FP64 kernels round every
+operation as the oracle does).  FP64 results do not depend on the launch
+configuration, so any difference means an indexing, chunking or
+synchronisation fault in one configuration.  FP64 Cash-Karp stages refuse
+the (4,4) and (8,2) tiles (variants 0 and 1) and keep their default there.
+The draws are fixed by the seed; the sweep is ordinary regression
+coverage."""
 import os
 import sys
 
@@ -49,13 +52,15 @@ def test_fp64_parity_across_launch_configurations(oracle, case):
         want = oracle.solver(cfg, st, want, NT)
     with Simulation(cfg, psi) as sim:
         for k in _lib.STAGE_KINDS:
+            if k.startswith("ck_") and case["variant"] in (0, 1):
+                with pytest.raises(_lib.GpeError):
+                    sim.tune_set(k, case["variant"], case["zchunk"])
+                sim.tune_set(k, 4, case["zchunk"])
+                continue
             sim.tune_set(k, case["variant"], case["zchunk"])
         sim.advance(3)
         got = sim.download()
         s = sim.state()
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12
-    # rk45 forms dt from the error estimate, a difference of nearly equal
-    # Cash-Karp sums: the one-ulp hypot differences of |psi^2| (device math
-    # library against glibc) reach it at ~1e-13, so t and dt get 1e-12
-    assert s["t"] == pytest.approx(st.t, rel=1e-12, abs=1e-15)
-    assert s["dt"].real == pytest.approx(st.dt_re, rel=1e-12)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    assert s["t"] == st.t and s["dt"].real == st.dt_re and s["dt"].imag == st.dt_im

@@ -99,3 +99,27 @@ def test_pow_correctly_rounded_and_near_host_libm(shim):
     got, libm = _call(shim.ref_pow, x, y)
     assert np.array_equal(got, libm, equal_nan=True)
     assert got[0] == 0.5 and got[1] == 2.0
+
+
+def test_oracle_control_pow_modes(shim, oracle):
+    """The oracle's step-control pow: the C library's (the reference's) by
+    default; correctly rounded from quad precision in the mode that pins a
+    device step control.  Two independent correctly rounded
+    implementations (this header's double-double pow_cr and the oracle's
+    powq) agree bit for bit, and both differ from the library's pow on the
+    same rare arguments."""
+    rng = np.random.default_rng(11)
+    n = 100_000
+    x = np.concatenate([np.exp(rng.uniform(np.log(1.89e-4), 0.0, n)), np.exp(rng.uniform(0.0, np.log(1e8), n))])
+    y = np.concatenate([np.full(n, -0.2), np.full(n, -0.25)])
+    cr, libm = _call(shim.ref_pow, x, y)
+    base = np.array([oracle.control_pow(a, b) for a, b in zip(x, y)])
+    assert np.array_equal(base, libm)
+    before = oracle.control_pow_ambiguous()
+    with oracle.correctly_rounded_control_pow():
+        q = np.array([oracle.control_pow(a, b) for a, b in zip(x, y)])
+    assert oracle.control_pow_ambiguous() == before
+    assert np.array_equal(q, cr)
+    assert 0 < np.count_nonzero(q != libm) < 2e-3 * len(x)
+    # the mode is restored on leaving the block: the library's pow again
+    assert np.array_equal(np.array([oracle.control_pow(a, b) for a, b in zip(x[:1000], y[:1000])]), libm[:1000])

@@ -12,7 +12,10 @@ Serving: the seeded Poisson stream of 40 requests/s (N in {48..256}, both
 BCs, both orders, 200-5000 steps, RK4 or RKF45, FP32C or FP64, ring /
 line-pair / random-phase states) for a short real-clock window, drained.
 Every delivered result must equal the same request run alone, bitwise, and
-the FP64 requests with N <= 64 must be within 1e-12 of the oracle.
+the FP64 requests with N <= 64 must equal the oracle bit for bit (rk45
+requests: the oracle with a correctly rounded step-control pow, as the
+device rounds it; the reference's C library pow is within 0.52 ulp, and
+over the 3536 steps of request 12 that moves psi by 5.1e-12 relative).
 """
 import os
 import sys
@@ -121,12 +124,18 @@ def test_serve_stream_window_equal_alone_and_oracle(oracle):
             continue
         t0 = time.perf_counter()
         st = oracle.make_state(r.cfg.dt)
-        want, _ = oracle.run(r.cfg, st, _device_initial_state(r), r.steps, NT)
+        # a served rk45 instance controls its step on the device, where pow
+        # is rounded correctly (the reference's C library pow is not): the
+        # oracle in that rounding mode is the bitwise reference
+        with oracle.correctly_rounded_control_pow():
+            want, _ = oracle.run(r.cfg, st, _device_initial_state(r), r.steps, NT)
         g, info = got[r.rid]
         e = rel_l2(g, want)
-        say("request %d (N=%d %s %s, %d steps) vs oracle rel L2 %.3e, t %.9g vs %.9g (%.0f s)" %
-            (r.rid, r.cfg.nx, r.cfg.scheme, r.ic_kind, r.steps, e, info["t"], st.t, time.perf_counter() - t0))
+        say("request %d (N=%d %s %s, %d steps) vs oracle rel L2 %.3e, bitwise %s, t %.17g vs %.17g (%.0f s)" %
+            (r.rid, r.cfg.nx, r.cfg.scheme, r.ic_kind, r.steps, e, np.array_equal(g, want), info["t"], st.t,
+             time.perf_counter() - t0))
         assert e < 1e-12
-        assert info["t"] == pytest.approx(st.t, rel=1e-13)
+        assert np.array_equal(g.view(np.uint64), want.view(np.uint64))
+        assert info["t"] == st.t
         checked += 1
     assert checked >= 2
