@@ -11,12 +11,19 @@ control and psi update) is the headline `value`; the plain FP64 path runs
 on the same input beside it.
 
 `value` is site-updates per second over the K timed steps, field resident
-in HBM.  `roofline` is for the dominant kernel, timed with HIP events on the
-stream it is launched on; its algorithmic bytes are the stage's share of the
-fused-sweep traffic model (RKF45: 27 field passes per attempt, + the low
-word read and write of the compensated form).  `cpu_baseline` is the C
-oracle (a port of the reference's FP64 path) timed on this host on one
-accepted step of the same 512^3 field.
+in HBM.  `roofline` is for the dominant kernel, timed with HIP events
+recorded on the stream it is launched on over the same K steps; its
+algorithmic bytes are the stage's share of the fused-sweep traffic model
+(RKF45: 27 field passes per attempt, + the low word read and write of the
+compensated form); `traffic` is the kernel's HBM bytes per launch from the
+committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary.  `long_run` advances the
+FP32C field 1000 accepted steps from t = 0 and reports its reject ratio.
+`cpu_baseline` is the C oracle (a port of the reference's FP64 path) timed
+on this host on the same 512^3 field, 3 steps minus 1.  The batch leg runs
+64 uneven simulations against the same ones run one after another in plain
+FP64; the serve leg feeds the 40 requests/s Poisson stream for 120 s and
+then runs a bounded sample of the completed requests one by one in plain
+FP64.
 """
 import argparse
 import json
@@ -57,18 +64,16 @@ def run_path(precision, psi, steps, warmup, tune=True):
     log(precision, "warmup", warmup)
     sim.advance(warmup)
     s0 = sim.state()
-    sim.prof_enable(False)
-    # timed region: the advance call synchronises the stream before returning
+    # timed region, with HIP events recorded around every stage launch on
+    # the context's stream (the kernel times of the roofline); the advance
+    # call synchronises the stream before returning
+    sim.prof_enable(True)
     t0 = time.perf_counter()
     sim.advance(steps)
     t1 = time.perf_counter()
+    prof = sim.prof_query()
     s1 = sim.state()
     elapsed = t1 - t0
-    # second pass with per-kernel HIP events (not part of the timed value)
-    sim.prof_enable(True)
-    sim.advance(max(2, min(steps, 10)))
-    prof = sim.prof_query()
-    s2 = sim.state()
     tiles = {k: v for k, v in sim.tune_get().items() if k in prof}
     sim.close()
     sites = cfg.nx * cfg.ny * cfg.nz
@@ -79,11 +84,31 @@ def run_path(precision, psi, steps, warmup, tune=True):
         "t_sim": s1["t"] - s0["t"],
         "dt_last": s1["dt"].real,
         "prof": prof,
-        "rejects_last": s2["n_rejects"],
         "sites": sites,
         "tiles": tiles,
-        "attempts_per_step": _attempts(prof) / max(2, min(steps, 10)),
+        "attempts_per_step": _attempts(prof) / steps,
     }
+
+
+def long_run(precision, psi, steps):
+    """The same field advanced `steps` accepted RKF45 steps from t = 0 (the
+    stored launch configuration): site-updates/s over the whole run and its
+    rejected-attempt ratio, which a short window from the initial field
+    does not show (dt first grows for a few hundred steps)."""
+    from gpe_amd.sim import Simulation
+    cfg = workload(precision)
+    with Simulation(cfg, psi) as sim:
+        sim.prof_enable(True)
+        t0 = time.perf_counter()
+        sim.advance(steps)
+        el = time.perf_counter() - t0
+        attempts = _attempts(sim.prof_query())
+        st = sim.state()
+    sites = cfg.nx * cfg.ny * cfg.nz
+    return {"precision": precision, "steps": steps, "attempts": attempts,
+            "reject_ratio": round((attempts - steps) / attempts, 4) if attempts else None,
+            "su_per_s": round(sites * steps / el, 1), "ms_per_step": round(1e3 * el / steps, 4),
+            "t_end": st["t"]}
 
 
 def _attempts(prof):
@@ -125,18 +150,39 @@ def step_roofline(res, bytes_per_site_attempt):
     return round(bytes_ / (tot_ms / 1e3) / 1e9, 1)
 
 
+def host_cpu():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
 def cpu_baseline(psi):
+    """The oracle (a C port of the reference's FP64 path) on this host's
+    cores: the same 512^3 field advanced 1 and then 3 accepted steps; the
+    difference of the two times is two steps without setup."""
     from oracle import pyoracle
     cfg = workload("fp64")
     nt = min(16, os.cpu_count() or 1)
-    st = pyoracle.make_state(cfg.dt)
-    t0 = time.perf_counter()
-    pyoracle.solver(cfg, st, psi, nt)
-    el = time.perf_counter() - t0
+    times = []
+    for k in (1, 3):
+        st = pyoracle.make_state(cfg.dt)
+        t0 = time.perf_counter()
+        out, _ = pyoracle.run(cfg, st, psi, k, nt)
+        times.append(time.perf_counter() - t0)
+        del out
+    el = times[1] - times[0]
     sites = cfg.nx * cfg.ny * cfg.nz
-    return {"value": round(sites / el, 1), "unit": "site-updates/s", "cores": nt, "kind": "port",
-            "sample": "one accepted rk45 step (%d attempt(s)) of the same 512^3 field, FP64, %.1f s"
-                      % (1 + st.n_rejects, el)}
+    return {"value": round(2 * sites / el, 1), "unit": "site-updates/s", "cores": nt, "kind": "port",
+            "host_cpu": host_cpu(), "host_threads": os.cpu_count(),
+            "sample": "the same 512^3 field, FP64: 3 accepted rk45 steps minus 1 (%.1f s - %.1f s = %.1f s "
+                      "for 2 steps, setup differenced out)" % (times[1], times[0], el)}
 
 
 def batch_leg(seed, steps=None, log_=log):
@@ -193,14 +239,50 @@ def batch_leg(seed, steps=None, log_=log):
             "psi_rel_l2_vs_fp64_max": max(errs), "tiles": tiles}
 
 
-def serve_leg(rate, duration, seed, log_=log):
-    """Poisson request stream into the continuous-batching server."""
+def seq_fp64(reqs, budget_s, log_=log):
+    """The requests run one after another on the plain FP64 single-simulation
+    path (the comparator of the served mix), until budget_s of timed work."""
+    from gpe_amd.sim import Simulation
+    el, su, n = 0.0, 0.0, 0
+    for r in reqs:
+        c = r.cfg.with_(precision="fp64")
+        with Simulation(c) as sim:
+            if r.ic_kind == "random_phase":
+                sim.random_phase(r.seed, kmax=0.5)
+            else:
+                sim.upload(r.initial_state())
+            t0 = time.perf_counter()
+            sim.advance(r.steps)
+            el += time.perf_counter() - t0
+        su += float(c.nx * c.ny * c.nz) * r.steps
+        n += 1
+        if el >= budget_s:
+            break
+    log_("sequential fp64: %d of the completed requests in %.1f s" % (n, el))
+    return {"requests": n, "elapsed_s": round(el, 3), "solves_per_s": round(n / el, 4) if el else None,
+            "site_updates_per_s": round(su / el, 1) if el else None}
+
+
+def serve_leg(rate, duration, seed, seq_budget_s=30.0, log_=log):
+    """Poisson request stream into the continuous-batching server, then the
+    completed requests one by one on the plain FP64 path (bounded sample)."""
     from gpe_amd.serve import Server
     from gpe_amd.workloads import request_stream
     reqs = request_stream(rate, duration, seed)
     with Server(max_sites=64e6) as s:
         m = s.serve(reqs, duration=duration, drain=False, log=lambda *x: log_("serve", *x))
+        done = [d[0] for d in s.last_done if d[4] == 1]
+        model_bytes = s.last_model_bytes
     m.update({"rate": rate, "duration_s": duration, "requests": len(reqs)})
+    # bytes of the fused-sweep model over all resident work (accepted steps;
+    # rk45 rejections not counted, so a lower bound on the bytes moved)
+    m["achieved_gbs_model"] = round(model_bytes / m["elapsed_s"] / 1e9, 1) if m["elapsed_s"] else None
+    if seq_budget_s > 0 and done:
+        seq = seq_fp64(done, seq_budget_s, log_)
+        m["seq_fp64"] = seq
+        if seq["site_updates_per_s"]:
+            m["served_over_seq_fp64_site_updates"] = round(m["site_updates_per_s"] / seq["site_updates_per_s"], 3)
+            m["served_over_seq_fp64_solves"] = round(m["solves_per_s"] / seq["solves_per_s"], 3)
     return m
 
 
@@ -212,10 +294,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-simulation batch leg")
-    ap.add_argument("--serve-seconds", type=float, default=30.0,
+    ap.add_argument("--serve-seconds", type=float, default=120.0,
                     help="length of the Poisson serving leg (0 to skip)")
     ap.add_argument("--only-batch", action="store_true",
                     help="run only the batch leg and print its JSON (for profiling)")
+    ap.add_argument("--long-steps", type=int, default=1000,
+                    help="accepted steps of the FP32C long run (reject ratio; 0 to skip)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -235,6 +319,10 @@ def main():
     log("fp32c %.3e su/s, %.3f ms/step" % (r32["su_per_s"], r32["ms_per_step"]))
     r64 = run_path("fp64", psi, a.steps, a.warmup)
     log("fp64  %.3e su/s, %.3f ms/step" % (r64["su_per_s"], r64["ms_per_step"]))
+    lr = None
+    if a.long_steps > 0:
+        lr = long_run("fp32c", psi, a.long_steps)
+        log("fp32c long run: %d steps, %d attempts, %.3e su/s" % (lr["steps"], lr["attempts"], lr["su_per_s"]))
 
     del psi
     bat = None if a.no_batch else batch_leg(a.seed)
@@ -269,6 +357,7 @@ def main():
         "hbm_probe_gbs": {k: round(v, 1) for k, v in probe.items()},
         "sim_time": {"fp32c": round(r32["t_sim"], 6), "fp64": round(r64["t_sim"], 6)},
         "attempts_per_step": {"fp32c": r32["attempts_per_step"], "fp64": r64["attempts_per_step"]},
+        "long_run": lr,
         "tiles": {"fp32c": r32["tiles"], "fp64": r64["tiles"]},
         "step_gbs": {"fp32c": step_roofline(r32, 232.0), "fp64": step_roofline(r64, 432.0)},
         "kernels": {"fp32c": {k: [v[0], round(v[1], 3)] for k, v in r32["prof"].items()},

@@ -59,6 +59,8 @@ EXPORTS = [
     "gpe_batch_remove", "gpe_batch_run", "gpe_batch_sync", "gpe_batch_poll", "gpe_batch_query",
     "gpe_batch_download", "gpe_batch_stats", "gpe_batch_use_graph", "gpe_batch_tune_set",
     "gpe_batch_tune_get", "gpe_batch_autotune",
+    "gpe_serve_create", "gpe_serve_destroy", "gpe_serve_submit", "gpe_serve_step", "gpe_serve_poll",
+    "gpe_serve_result",
 ]
 
 
@@ -163,6 +165,13 @@ def lib():
         "gpe_batch_tune_set": ([vp, i32, i32, i32, i32, i32, i32], ctypes.c_int),
         "gpe_batch_tune_get": ([vp, i32, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)], ctypes.c_int),
         "gpe_batch_autotune": ([vp, i32], ctypes.c_int),
+        "gpe_serve_create": ([i64, i64, ctypes.POINTER(vp)], ctypes.c_int),
+        "gpe_serve_destroy": ([vp], None),
+        "gpe_serve_submit": ([vp, P, vp, i32, ctypes.c_uint64, ctypes.c_double, S, i64, ctypes.POINTER(i64)],
+                             ctypes.c_int),
+        "gpe_serve_step": ([vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)], ctypes.c_int),
+        "gpe_serve_poll": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i32), i32, ctypes.POINTER(i32)], ctypes.c_int),
+        "gpe_serve_result": ([vp, i64, vp, i32, S], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -30,7 +30,10 @@ UNITS = [("gpe_misc.hip", "gpe_misc", ["-ffp-contract=off"]),
          ("host_common.cpp", "host_common", ["-ffp-contract=off", "-x", "hip"]),
          ("gpe_ic.cpp", "gpe_ic", ["-ffp-contract=off", "-x", "hip"]),
          ("tune_table.cpp", "tune_table", ["-x", "hip"]),
-         ("gpe_spectral.cpp", "gpe_spectral", ["-ffp-contract=off", "-x", "hip"])]
+         ("gpe_spectral.cpp", "gpe_spectral", ["-ffp-contract=off", "-x", "hip"]),
+         ("gpe_serve.cpp", "gpe_serve", ["-x", "hip"]),
+         ("errors.cpp", "errors", ["-x", "hip"]),
+         ("ranges.cpp", "ranges", ["-x", "hip"])]
 for _v in range(N_VARIANTS):
     # stage outputs are written with nontemporal stores (-DGPE_NT_STORE): they
     # are not re-read within a sweep, and keeping them out of L2 leaves room
@@ -74,7 +77,7 @@ def build(verbose=True, extra_flags=(), lib_path=None, obj_dir=None):
     # hipFFT (the random-phase initial state) from the ROCm install
     rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "lib")
     cmd = ([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
-           ["-L" + rocm_lib, "-lhipfft", "-Wl,-rpath," + rocm_lib])
+           ["-L" + rocm_lib, "-lhipfft", "-lrocprofiler-sdk-roctx", "-Wl,-rpath," + rocm_lib])
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))

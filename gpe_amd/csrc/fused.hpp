@@ -165,7 +165,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
     } else {
       cx<T> u = r.ps;
 #pragma unroll
-      for (int m = 0; m < NIN; ++m) u = u + rmul((T)A.b[m], r.kk[m]);
+      for (int m = 0; m < NIN; ++m) u = axpy<COMP>((T)A.b[m], r.kk[m], u);
       return u;
     }
   };
@@ -176,7 +176,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
     } else {
       cx<T> u = r.ps;
 #pragma unroll
-      for (int m = 0; m < NIN; ++m) u = u + rmul((T)B.b[m], r.kk[m]);
+      for (int m = 0; m < NIN; ++m) u = axpy<COMP>((T)B.b[m], r.kk[m], u);
       return u;
     }
   };
@@ -184,7 +184,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
     if constexpr (KindInfo<KB>::complex_coef)
       return w + cmul(HB, ka);
     else
-      return w + rmul((T)B.b[NIN], ka);
+      return axpy<COMP>((T)B.b[NIN], ka, w);
   };
   // B's final-combination partials known at load time
   auto make_F = [&](const Raw &r, cx<T> *F) {
@@ -194,11 +194,11 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
     } else if constexpr (KB == K_CK_6) {
       cx<T> e = rmul((T)B.dc[1], r.kk[0]);
 #pragma unroll
-      for (int m = 1; m < 4; ++m) e = e + rmul((T)B.dc[m + 1], r.kk[m]);
+      for (int m = 1; m < 4; ++m) e = axpy<COMP>((T)B.dc[m + 1], r.kk[m], e);
       if constexpr (COMP) {
         cx<T> pd = rmul((T)B.c[1], r.kk[0]);  // c2 = c5 = 0
-        pd = pd + rmul((T)B.c[3], r.kk[2]);
-        pd = pd + rmul((T)B.c[4], r.kk[3]);
+        pd = axpy<COMP>((T)B.c[3], r.kk[2], pd);
+        pd = axpy<COMP>((T)B.c[4], r.kk[3], pd);
         F[0] = r.ps;
         F[1] = pd;
         F[2] = e;
@@ -220,7 +220,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
       F[1] = F[1] + rdiv(ka, B.d3, 1.0 / 3.0);
     } else if constexpr (KB == K_CK_6) {
       if constexpr (COMP) {
-        F[2] = F[2] + rmul((T)B.dc[5], ka);
+        F[2] = axpy<COMP>((T)B.dc[5], ka, F[2]);
       } else {
         F[0] = F[0] + rmul((T)B.c[5], ka);
         F[1] = F[1] + rmul((T)B.dc[5], ka);
@@ -517,13 +517,15 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
                 const cx<T> hi = mk((T)np.re, (T)np.im);
                 stc<T>(B.psi_out, s, hi);
                 stc<T>(B.lo_out, s, mk((T)(np.re - (double)hi.re), (T)(np.im - (double)hi.im)));
+                const double e2 = err_ratio2(err, sc);  // |err/scale|^2, as the one-stage kernel
+                acc_emax = (e2 > acc_emax || e2 != e2) ? e2 : acc_emax;
               } else {
                 const cx<T> nf = F[0] + rmul((T)B.c[6], kk);
                 stc<T>(B.psi_out, s, nf);
                 np = todbl(nf);
+                const float e = cabs_(cdiv(err, sc));  // abs(err/scale) (solve.f90:184)
+                acc_emax = ((double)e > acc_emax || e != e) ? (double)e : acc_emax;
               }
-              const float e = cabs_(cdiv(err, sc));  // abs(err/scale) (solve.f90:184)
-              acc_emax = ((double)e > acc_emax || e != e) ? (double)e : acc_emax;
             }
             const double wn = nweight(gi, nx, B.w1, B.w2, B.w3) * nweight(gj[p], ny, B.w1, B.w2, B.w3) *
                               nweight(k, nz, B.w1, B.w2, B.w3);
@@ -571,6 +573,8 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
   }
   static_assert(D >= 1 && D <= 4, "prefetch depth 1..4");
 
+  // the compensated form maximised |err/scale|^2 (as the one-stage kernel)
+  if constexpr (KB == K_CK_6 && COMP) acc_emax = sqrt(acc_emax);
   if constexpr (KindInfo<KB>::final_) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {

@@ -32,9 +32,9 @@ using namespace gpe;
 
 namespace {
 
-constexpr int kDefaultVid = 0;
+constexpr int kBatchDefaultVid = 0;
 // the default tile variant of stage kind k in a group of precision prec
-int default_vid(int prec, int k) { return variant_allowed(prec == GPE_FP64, k, kDefaultVid) ? kDefaultVid : kVidF64Ck; }
+int default_vid(int prec, int k) { return variant_allowed(prec == GPE_FP64, k, kBatchDefaultVid) ? kBatchDefaultVid : kVidF64Ck; }
 constexpr int kDefaultZc = 32;
 constexpr int kMaxGrid = 2048;  // persistent workgroups per batched launch
 
@@ -769,6 +769,7 @@ int gpe_batch_remove(gpe_batch *b, int32_t id) {
 
 int gpe_batch_run(gpe_batch *b, int32_t iterations) {
   if (!b) return fail(GPE_EINVAL, "batch is NULL");
+  Range range("batch run");
   if (iterations < 0) return fail(GPE_EINVAL, "iterations must be >= 0");
   if (b->id2slot.empty()) return GPE_OK;
   int rc = prepare(b);  // iterations == 0: only rebuild the tables and capture

@@ -145,28 +145,7 @@ struct gpe_ctx {
 
 namespace {
 
-int grid_for_v(gpe_ctx *c, int vid, int zc, StageArgs &A) {
-  const Tile t = kTiles[vid];
-  const int TY = t.nw * t.p;
-  A.ntx = (c->P.nx + 63) / 64;
-  A.nty = (c->P.ny + TY - 1) / TY;
-  const int tiles2d = A.ntx * A.nty;
-  if (zc <= 0) {
-    // enough workgroups to fill 256 CUs several times over, but z-chunks
-    // long enough that re-reading 2R planes per chunk stays small: at least
-    // 32 planes unless that leaves fewer than 1024 workgroups (small grids,
-    // where the z march's latency, not its traffic, sets the time)
-    int nzc = (2048 + tiles2d - 1) / tiles2d;
-    zc = (c->P.nz + nzc - 1) / nzc;
-    if (zc < 32 && tiles2d * ((c->P.nz + 31) / 32) >= 1024) zc = 32;
-  }
-  if (zc < 8) zc = 8;
-  if (zc > c->P.nz) zc = c->P.nz;
-  A.zc = zc;
-  A.nzc = (c->P.nz + zc - 1) / zc;
-  A.nblocks = tiles2d * A.nzc;
-  return A.nblocks;
-}
+int grid_for_v(gpe_ctx *c, int vid, int zc, StageArgs &A) { return stage_grid(c->P.nx, c->P.ny, c->P.nz, vid, zc, A); }
 
 int grid_for(gpe_ctx *c, int kind, StageArgs &A) { return grid_for_v(c, c->vid[kind], c->zc[kind], A); }
 
@@ -217,6 +196,7 @@ void prof_drain(gpe_ctx *c) {
 }
 
 int run_kind(gpe_ctx *c, int kind, StageArgs &A) {
+  Range range(kind_name(kind));
   grid_for(c, kind, A);
   if (A.nblocks > c->part_cap) return fail(GPE_EINVAL, "internal: %d workgroups exceed partial buffer", A.nblocks);
   hipEvent_t ea = nullptr, eb = nullptr;
@@ -236,6 +216,7 @@ int run_kind(gpe_ctx *c, int kind, StageArgs &A) {
 }
 
 int run_pair(gpe_ctx *c, int pair, const StageArgs &A, StageArgs &B) {
+  Range range(kind_name(KP0 + pair));
   grid_for_pair_v(c, c->pvid[pair], c->pzc[pair], B);
   if (B.nblocks > c->part_cap) return fail(GPE_EINVAL, "internal: %d workgroups exceed partial buffer", B.nblocks);
   hipEvent_t ea = nullptr, eb = nullptr;
@@ -398,6 +379,7 @@ int rkqs(gpe_ctx *c, double *hist_slot) {
   int rejects = 0;
   int rc;
   for (int attempt = 0;; ++attempt) {
+    Range range("rk45 attempt");
     c->st.dt_re = (double)dtr;
     c->st.dt_im = (double)dti;
     // descriptor of Cash-Karp stage s (kk_s = dt f(psi + sum_m b_sm kk_m))
@@ -485,6 +467,7 @@ int rkqs(gpe_ctx *c, double *hist_slot) {
 }
 
 int solver_call(gpe_ctx *c, double *hist_slot) {
+  Range range("solver");
   if (c->P.scheme == GPE_RK45) return c->prec == GPE_FP32 ? rkqs<float>(c, hist_slot) : rkqs<double>(c, hist_slot);
   return step_once(c, hist_slot);
 }
@@ -578,7 +561,7 @@ int gpe_ctx_create(const gpe_params *params, gpe_ctx **out) {
     // default launch configuration; partial buffers sized for the largest
     // grid any variant and z-chunk can produce
     for (int kk = 0; kk < K_COUNT; ++kk) {
-      c->vid[kk] = variant_allowed(c->prec == GPE_FP64, kk, 1) ? 1 : kVidF64Ck;
+      c->vid[kk] = variant_allowed(c->prec == GPE_FP64, kk, kDefaultVid) ? kDefaultVid : kVidF64Ck;
       c->zc[kk] = 0;
     }
     // every pair runs as its two stages until the autotuner (or

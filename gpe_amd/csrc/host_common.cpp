@@ -1,6 +1,7 @@
 // host_common.cpp -- host helpers shared by the single-simulation and the
-// batch entry points: error reporting, the launch descriptor's constant
-// part, and field conversions between host arrays and device storage.
+// batch entry points: the launch descriptor's constant part and field
+// conversions between host arrays and device storage (error reporting is
+// in errors.cpp).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -10,22 +11,6 @@
 #include <vector>
 
 #include "host_common.h"
-
-namespace gpe {
-
-thread_local std::string g_err;
-
-int fail(int code, const char *fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return code;
-}
-
-}  // namespace gpe
 
 namespace gpe {
 // Division by a real constant in the reference's form (see stage_args.h).

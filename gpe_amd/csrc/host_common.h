@@ -30,6 +30,16 @@ int field_upload(int prec, size_t plane, int nz, const void *src, int host_type,
 int field_download(int prec, size_t plane, int nz, const void *hi, const void *lo, void *dst, int host_type);
 extern thread_local std::string g_err;
 
+// A roctx range for the lifetime of the object (rocprofv3 --marker-trace
+// shows one per solver() call, rk45 attempt, stage launch and batch run;
+// without a profiler attached a push/pop costs a few tens of ns).
+struct Range {
+  explicit Range(const char *name);
+  ~Range();
+  Range(const Range &) = delete;
+  Range &operator=(const Range &) = delete;
+};
+
 // One stored launch configuration: stage-kind name, tile variant, z-chunk.
 struct TuneItem {
   std::string name;

@@ -30,6 +30,35 @@ constexpr bool variant_allowed(bool fp64, int kind, int vid) {
   return vid >= 0 && vid < N_TILE_VARIANTS && !(fp64 && kind >= K_CK_1 && kind <= K_CK_6 && vid <= 1);
 }
 
+// Tile variant of a new context's stage kinds (the (8,2) tile), where allowed.
+constexpr int kDefaultVid = 1;
+
+// Tiling of one stage sweep over an nx x ny x nz grid with tile variant vid
+// and z-chunk zc (0: automatic): sets A.ntx, A.nty, A.zc, A.nzc and
+// A.nblocks and returns the number of workgroups.  Automatic chunks give
+// enough workgroups to fill 256 CUs several times over, but stay long
+// enough that re-reading 2R planes per chunk stays small: at least 32
+// planes unless that leaves fewer than 1024 workgroups (small grids, where
+// the z march's latency, not its traffic, sets the time).
+inline int stage_grid(int nx, int ny, int nz, int vid, int zc, StageArgs &A) {
+  const Tile t = kTiles[vid];
+  const int TY = t.nw * t.p;
+  A.ntx = (nx + 63) / 64;
+  A.nty = (ny + TY - 1) / TY;
+  const int tiles2d = A.ntx * A.nty;
+  if (zc <= 0) {
+    int nzc = (2048 + tiles2d - 1) / tiles2d;
+    zc = (nz + nzc - 1) / nzc;
+    if (zc < 32 && tiles2d * ((nz + 31) / 32) >= 1024) zc = 32;
+  }
+  if (zc < 8) zc = 8;
+  if (zc > nz) zc = nz;
+  A.zc = zc;
+  A.nzc = (nz + zc - 1) / zc;
+  A.nblocks = tiles2d * A.nzc;
+  return A.nblocks;
+}
+
 // Launch stage kernel `kind` (radius 1 or 2) over A.nblocks workgroups with
 // tile variant `vid`.  comp selects the compensated FP32 form.
 hipError_t launch_stage(int prec_bytes, int vid, int kind, int radius, bool comp, const StageArgs &A,

@@ -42,23 +42,24 @@ namespace gpe {
 namespace refm {
 
 // --- hypot ------------------------------------------------------------------
-GPE_HD double hypot_kernel(double ax, double ay) {  // ax >= ay, both in range
-  double h = std::sqrt(ax * ax + ay * ay);
-  double t1, t2;
-  if (h <= 2.0 * ay) {
-    const double delta = h - ay;
-    t1 = ax * (2.0 * delta - ax);
-    t2 = (delta - 2.0 * (ax - ay)) * delta;
-  } else {
-    const double delta = h - ax;
-    t1 = 2.0 * delta * (ax - 2.0 * ay);
-    t2 = (4.0 * delta - ay) * ay + delta * delta;
-  }
-  h -= (t1 + t2) / (2.0 * h);
-  return h;
+// ax >= ay > 0, both in range: h = sqrt(ax^2 + ay^2), then the correction.
+// Both correction formulas are evaluated and one is selected (no divergent
+// branch on the device); each is the same expression either way.
+GPE_HD double hypot_kernel(double ax, double ay) {
+  const double h = std::sqrt(ax * ax + ay * ay);
+  const bool near = h <= 2.0 * ay;
+  const double d1 = h - ay, d2 = h - ax;
+  const double t1a = ax * (2.0 * d1 - ax);
+  const double t2a = (d1 - 2.0 * (ax - ay)) * d1;
+  const double t1b = 2.0 * d2 * (ax - 2.0 * ay);
+  const double t2b = (4.0 * d2 - ay) * ay + d2 * d2;
+  const double t1 = near ? t1a : t1b, t2 = near ? t2a : t2b;
+  return h - (t1 + t2) / (2.0 * h);
 }
 
-GPE_HD double hypot_ref(double x, double y) {
+// Operands out of the kernel's range: non-finite, very large or very small,
+// or so different in size that the smaller one does not count.
+GPE_HD double hypot_edge(double x, double y) {
   constexpr double kScale = 0x1p-600, kLarge = 0x1p+511, kTiny = 0x1p-511, kEps = 0x1p-54;
   if (!std::isfinite(x) || !std::isfinite(y)) {
     if (std::isinf(x) || std::isinf(y)) return INFINITY;
@@ -75,8 +76,16 @@ GPE_HD double hypot_ref(double x, double y) {
     if (ax >= ay / kEps) return ax + ay;
     return hypot_kernel(ax / kScale, ay / kScale) * kScale;
   }
-  if (ay <= ax * kEps) return ax + ay;
-  return hypot_kernel(ax, ay);
+  return ax + ay;  // ay <= ax * kEps
+}
+
+GPE_HD double hypot_ref(double x, double y) {
+  constexpr double kLarge = 0x1p+511, kTiny = 0x1p-511, kEps = 0x1p-54;
+  const double fx = std::fabs(x), fy = std::fabs(y);
+  const double ax = fx < fy ? fy : fx, ay = fx < fy ? fx : fy;
+  // false for NaN and infinite operands, which take the edge path
+  if (ax <= kLarge && ay >= kTiny && ay > ax * kEps) return hypot_kernel(ax, ay);
+  return hypot_edge(x, y);
 }
 
 // --- double-double arithmetic (Dekker 1971; Knuth TAOCP vol. 2, 4.2.2) -----

@@ -137,6 +137,35 @@ __device__ __forceinline__ cx<float> cdiv(cx<float> a, cx<float> b) {
 __device__ __forceinline__ double cabs_(cx<double> a) { return refm::hypot_ref(a.re, a.im); }
 __device__ __forceinline__ float cabs_(cx<float> a) { return hypotf(a.re, a.im); }
 
+// y + b*x.  The compensated FP32 form (FMA) rounds once per component with an
+// explicit fused multiply-add, so every instantiation of a stage (single,
+// batched, fused pair) gives the same bits; otherwise the product and the sum
+// round separately, in the reference's order.
+template <bool FMA, typename T>
+__device__ __forceinline__ cx<T> axpy(T b, cx<T> x, cx<T> y) {
+  if constexpr (FMA && std::is_same<T, float>::value) {
+    return mk(fmaf(b, x.re, y.re), fmaf(b, x.im, y.im));
+  } else {
+    return y + rmul(b, x);
+  }
+}
+
+// |a| of an FP32 value through FP64 (no underflow of |a|^2) and the
+// hardware square root (about 1 ulp): for the compensated form's error scale,
+// which only sets the tolerance.
+__device__ __forceinline__ float mag_fast(cx<float> a) {
+  const double r = (double)a.re, i = (double)a.im;
+  return (float)__builtin_amdgcn_sqrt(fma(r, r, i * i));
+}
+
+// |err/scale|^2 in FP64 (no underflow at the 1e-30 floor of the scale): the
+// compensated form's rk45 error measure; the maximum's square root is taken
+// once per thread.
+__device__ __forceinline__ double err_ratio2(cx<float> e, cx<float> sc) {
+  const double er = (double)e.re, ei = (double)e.im, sr = (double)sc.re, si = (double)sc.im;
+  return fma(er, er, ei * ei) / fma(sr, sr, si * si);
+}
+
 // Index map for a ghost index i in [-R, n-1+R].
 __device__ __forceinline__ int bcmap(int i, int n, int periodic) {
   if (periodic) return i < 0 ? i + n : (i >= n ? i - n : i);
@@ -275,10 +304,9 @@ struct FifoInfo {
 // (|f| = |kk1| / |dt|), instead of storing it in stage 1 and reading it
 // back: one field pass less per attempt.  FP64 and plain FP32 store it as
 // the reference computes it (solve.f90:176-178).
-template <typename T>
-__device__ __forceinline__ cx<T> scale_from_kk1(const StageArgs &A, cx<T> ps, cx<T> kk1) {
-  const T af = cabs_(kk1) * (T)A.inv_absdt, au = cabs_(ps);
-  return mk((au + (T)A.sdtr * af) + (T)1e-30, (T)A.sdti * af);
+__device__ __forceinline__ cx<float> scale_from_kk1(const StageArgs &A, cx<float> ps, cx<float> kk1) {
+  const float af = mag_fast(kk1) * (float)A.inv_absdt, au = mag_fast(ps);
+  return mk(fmaf((float)A.sdtr, af, au) + 1e-30f, (float)A.sdti * af);
 }
 
 // Pointwise fields read only at the output plane: the rk45 error scale and
@@ -379,6 +407,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
   }
 
   const cx<T> H = KIND == K_RK4_4 ? mk((T)A.dtr, (T)A.dti) : mk((T)A.hr, (T)A.hi);
+  // get_norm weights of this thread's column and rows (constant over the
+  // z march); the product wx*wy*wz is formed in the same order as before
+  const double wx = nweight(gi, nx, A.w1, A.w2, A.w3);
+  double wy[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) wy[p] = nweight(gj[p], ny, A.w1, A.w2, A.w3);
 
   auto load_raw = [&](size_t s, Raw &r, const bool full) {
     r.ps = ldc<T>(A.psi, s);
@@ -399,7 +433,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
     } else {
       cx<T> u = r.ps;
 #pragma unroll
-      for (int m = 0; m < NIN; ++m) u = u + rmul((T)A.b[m], r.kk[m]);
+      for (int m = 0; m < NIN; ++m) u = axpy<COMP>((T)A.b[m], r.kk[m], u);
       return u;
     }
   };
@@ -414,11 +448,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
     } else if constexpr (KIND == K_CK_6) {
       cx<T> pe = rmul((T)A.dc[1], r.kk[0]);
 #pragma unroll
-      for (int m = 1; m < 5; ++m) pe = pe + rmul((T)A.dc[m + 1], r.kk[m]);
+      for (int m = 1; m < 5; ++m) pe = axpy<COMP>((T)A.dc[m + 1], r.kk[m], pe);
       if constexpr (COMP) {
         cx<T> pd = rmul((T)A.c[1], r.kk[0]);  // c2 = c5 = 0
-        pd = pd + rmul((T)A.c[3], r.kk[2]);
-        pd = pd + rmul((T)A.c[4], r.kk[3]);
+        pd = axpy<COMP>((T)A.c[3], r.kk[2], pd);
+        pd = axpy<COMP>((T)A.c[4], r.kk[3], pd);
         F[0] = r.ps;
         F[1] = pd;
         F[2] = pe;
@@ -441,7 +475,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
       else
         o[i++] = ldc<T>(A.scale_in, s);
     }
-    if constexpr (COMP && KI::final_) o[i++] = ldc<T>(A.lo, s);
+    if constexpr (COMP && KI::final_) {
+      // the low word is read at the tile's own sites only (no neighbour
+      // reads it as a ghost): stream it past L2
+      typedef T nv2 __attribute__((ext_vector_type(2)));
+      const nv2 v = __builtin_nontemporal_load(static_cast<const nv2 *>(A.lo) + s);
+      o[i++] = mk<T>(v.x, v.y);
+    }
     (void)i;
   };
 
@@ -633,17 +673,19 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
                   const cx<T> hi = mk((T)np.re, (T)np.im);
                   stc<T>(A.psi_out, s, hi);
                   stc<T>(A.lo_out, s, mk((T)(np.re - (double)hi.re), (T)(np.im - (double)hi.im)));
+                  // |err/scale|^2, maximised; its square root after the march
+                  const double e2 = err_ratio2(err, sc);
+                  acc_emax = (e2 > acc_emax || e2 != e2) ? e2 : acc_emax;
                 } else {
                   const cx<T> nf = F[0][p][0] + rmul((T)A.c[6], kk);
                   stc<T>(A.psi_out, s, nf);
                   np = todbl(nf);
+                  // abs(err/scale) (solve.f90:184) in single precision
+                  const float e = cabs_(cdiv(err, sc));
+                  acc_emax = ((double)e > acc_emax || e != e) ? (double)e : acc_emax;
                 }
-                // abs(err/scale) (solve.f90:184) in single precision
-                const float e = cabs_(cdiv(err, sc));
-                acc_emax = ((double)e > acc_emax || e != e) ? (double)e : acc_emax;
               }
-              const double wn = nweight(gi, nx, A.w1, A.w2, A.w3) * nweight(gj[p], ny, A.w1, A.w2, A.w3) *
-                                nweight(k, nz, A.w1, A.w2, A.w3);
+              const double wn = wx * wy[p] * nweight(k, nz, A.w1, A.w2, A.w3);
               if constexpr (IS_DBL) {
                 const double mm = cabs_(np);
                 acc_norm += wn * (mm * mm);
@@ -678,8 +720,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
               stc<T>(A.psi_out, s, nf);
               np = todbl(nf);
             }
-            const double wn = nweight(gi, nx, A.w1, A.w2, A.w3) * nweight(gj[p], ny, A.w1, A.w2, A.w3) *
-                              nweight(k, nz, A.w1, A.w2, A.w3);
+            const double wn = wx * wy[p] * nweight(k, nz, A.w1, A.w2, A.w3);
             if constexpr (IS_DBL) {
               const double mm = cabs_(np);
               acc_norm += wn * (mm * mm);
@@ -717,6 +758,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
     }
   }
 
+  // The compensated form maximised |err/scale|^2: the maximum of |err/scale|
+  // is its square root (monotone; NaN stays NaN).
+  if constexpr (KIND == K_CK_6 && COMP) acc_emax = sqrt(acc_emax);
   // Deterministic block reductions: xor-shuffle within the wave, then the
   // waves in order.
   if constexpr (KI::final_ || KIND == K_ENERGY) {

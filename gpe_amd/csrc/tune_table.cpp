@@ -69,11 +69,17 @@ struct Cursor {
     s = e;
     return true;
   }
-  // Skip one value of any type.
-  bool skip() {
+  // Skip one value of any type (nested at most kMaxDepth deep, so a
+  // malformed file cannot exhaust the stack).
+  static constexpr int kMaxDepth = 64;
+  bool skip(int depth = 0) {
     ws();
     if (s >= end) {
       err = "unexpected end";
+      return false;
+    }
+    if (depth > kMaxDepth) {
+      err = "values nested too deep";
       return false;
     }
     if (*s == '"') {
@@ -93,7 +99,7 @@ struct Cursor {
           std::string k;
           if (!string(k) || !expect(':')) return false;
         }
-        if (!skip()) return false;
+        if (!skip(depth + 1)) return false;
         if (peek(',')) {
           ++s;
           continue;

@@ -31,6 +31,16 @@ from .sim import random_phase_prepare
 from .workloads import Request
 
 
+# Algorithmic bytes per site-update of the fused-sweep model (RK4 13 field
+# passes per step, RKF45 27 per attempt, counted once per accepted step; the
+# compensated form adds its low word's read and write).
+_BYTES = {("rk4", "fp64"): 208.0, ("rk4", "fp32c"): 120.0, ("rk45", "fp64"): 432.0, ("rk45", "fp32c"): 232.0}
+
+
+def _model_bytes(cfg):
+    return _BYTES.get((cfg.scheme, cfg.precision), 0.0)
+
+
 class _Ready:
     """A computed initial state kept for a request still in the queue."""
 
@@ -105,6 +115,7 @@ class Server:
         chunk = 4
         busy_s = 0.0
         su_done = 0.0
+        model_bytes = 0.0
         last_log = 0.0
 
         def on_device(r):
@@ -179,6 +190,7 @@ class Server:
                 resident_sites -= sites
                 assert resident_sites >= 0, resident_sites
                 su_done += float(sites) * q["steps"]
+                model_bytes += float(sites) * q["steps"] * _model_bytes(r.cfg)
                 done.append((r, r.arrival, tadm, tnow, status, q["steps"]))
                 if self.deliver is not None:
                     self.deliver(r, psi, q)
@@ -198,8 +210,15 @@ class Server:
         # work done on requests still running counts toward throughput
         su_partial = 0.0
         for iid, (r, _, sites) in running.items():
-            su_partial += float(sites) * self.batch.query(iid)["steps"]
+            k = float(sites) * self.batch.query(iid)["steps"]
+            su_partial += k
+            model_bytes += k * _model_bytes(r.cfg)
         m = self._metrics(done, queue, pending, running, end, busy_s, su_done + su_partial)
+        # for the caller: the finished requests (request, arrival, admitted,
+        # finished, status, steps) and the fused-sweep model bytes of all
+        # work done (accepted steps)
+        self.last_done = done
+        self.last_model_bytes = model_bytes
         m["site_updates_per_s_completed_only"] = su_done / end if end > 0 else 0.0
         return m
 
@@ -229,6 +248,84 @@ class Server:
             "latency_all_p50_s_lower_bound": pct(allat, 50), "latency_all_p99_s_lower_bound": pct(allat, 99),
             "device_busy_frac": busy_s / end if end > 0 else 0.0,
         }
+
+
+class NativeServer:
+    """The library's own serving loop (gpe_serve_* in the C ABI): the same
+    first-come-first-served admission by HBM footprint and resident-site cap
+    over one batch, with the scheduling in C++ so C and Fortran callers have
+    it too.  Python only submits, steps and collects."""
+
+    def __init__(self, hbm_budget=0, max_sites=0):
+        import ctypes
+        from . import _lib
+        self._c = ctypes
+        self._L = _lib.lib()
+        self._h = ctypes.c_void_p()
+        _lib.check(self._L.gpe_serve_create(int(hbm_budget), int(max_sites), ctypes.byref(self._h)))
+        self._req = {}
+
+    def close(self):
+        if self._h:
+            self._L.gpe_serve_destroy(self._h)
+            self._h = self._c.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, r, device_ic=True):
+        """Queue request r (a workloads.Request); returns its ticket.
+        Random-phase states are built on the device at admission."""
+        from . import _lib
+        from .sim import make_params
+        c = self._c
+        st = _lib.GpeStepState()
+        dt = complex(r.cfg.dt)
+        st.dt_re, st.dt_im = dt.real, dt.imag
+        t = c.c_int64()
+        if device_ic and r.ic_kind == "random_phase":
+            rc = self._L.gpe_serve_submit(self._h, c.byref(make_params(r.cfg)), None, _lib.GPE_HOST_C128,
+                                          int(r.seed), 0.5, c.byref(st), int(r.steps), c.byref(t))
+        else:
+            psi = np.ascontiguousarray(r.initial_state(), dtype=np.complex128)
+            rc = self._L.gpe_serve_submit(self._h, c.byref(make_params(r.cfg)), psi.ctypes.data_as(c.c_void_p),
+                                          _lib.GPE_HOST_C128, 0, 0.0, c.byref(st), int(r.steps), c.byref(t))
+        _lib.check(rc)
+        self._req[t.value] = r
+        return t.value
+
+    def step(self, iterations):
+        """One round: admit, run `iterations`, retire.  Returns (running, queued)."""
+        from . import _lib
+        c = self._c
+        run, q = c.c_int32(), c.c_int32()
+        _lib.check(self._L.gpe_serve_step(self._h, int(iterations), c.byref(run), c.byref(q)))
+        return run.value, q.value
+
+    def poll(self):
+        """[(ticket, status)] of requests finished since the last poll."""
+        from . import _lib
+        c = self._c
+        cap = max(1, len(self._req))
+        tk = (c.c_int64 * cap)()
+        stt = (c.c_int32 * cap)()
+        n = c.c_int32()
+        _lib.check(self._L.gpe_serve_poll(self._h, tk, stt, cap, c.byref(n)))
+        return [(tk[i], stt[i]) for i in range(n.value)]
+
+    def result(self, ticket):
+        """(field as complex128, state dict) of a finished request."""
+        from . import _lib
+        c = self._c
+        r = self._req.pop(ticket)
+        out = np.empty(r.cfg.shape, dtype=np.complex128)
+        st = _lib.GpeStepState()
+        _lib.check(self._L.gpe_serve_result(self._h, int(ticket), out.ctypes.data_as(c.c_void_p),
+                                            _lib.GPE_HOST_C128, c.byref(st)))
+        return out, {"t": st.t, "dt": complex(st.dt_re, st.dt_im), "im_t": st.im_t, "steps": st.p}
 
 
 def main():

@@ -28,14 +28,29 @@ def write_state(path, psi, p=0, t=0.0, im_t=0.0, dt=0.0):
         fh.write(psi.tobytes())
 
 
+def _read_exact(fh, n, what, path):
+    b = fh.read(n)
+    if len(b) != n:
+        raise ValueError("%s: truncated %s (%d of %d bytes)" % (path, what, len(b), n))
+    return b
+
+
 def read_state(path, single=False):
-    """Returns (psi, p, t, im_t, dt)."""
+    """Returns (psi, p, t, im_t, dt).  A file whose header does not describe
+    its own size raises ValueError before the field is read."""
     rt, ct = (np.float32, np.complex64) if single else (np.float64, np.complex128)
+    rs, cs = np.dtype(rt).itemsize, np.dtype(ct).itemsize
+    size = os.path.getsize(path)
     with open(path, "rb") as fh:
-        nx, ny, nz, p = (int(v) for v in np.frombuffer(fh.read(16), dtype=np.int32))
-        t, im_t = (float(v) for v in np.frombuffer(fh.read(2 * np.dtype(rt).itemsize), dtype=rt))
-        dt = complex(np.frombuffer(fh.read(np.dtype(ct).itemsize), dtype=ct)[0])
-        psi = np.frombuffer(fh.read(nx * ny * nz * np.dtype(ct).itemsize), dtype=ct).reshape(nz, ny, nx).copy()
+        nx, ny, nz, p = (int(v) for v in np.frombuffer(_read_exact(fh, 16, "header", path), dtype=np.int32))
+        if min(nx, ny, nz) < 1:
+            raise ValueError("%s: grid %dx%dx%d in the header" % (path, nx, ny, nz))
+        want = 16 + 2 * rs + cs + nx * ny * nz * cs
+        if size != want:
+            raise ValueError("%s: %d bytes, the header's %dx%dx%d grid needs %d" % (path, size, nx, ny, nz, want))
+        t, im_t = (float(v) for v in np.frombuffer(_read_exact(fh, 2 * rs, "times", path), dtype=rt))
+        dt = complex(np.frombuffer(_read_exact(fh, cs, "dt", path), dtype=ct)[0])
+        psi = np.frombuffer(_read_exact(fh, nx * ny * nz * cs, "field", path), dtype=ct).reshape(nz, ny, nx).copy()
     return psi, p, t, im_t, dt
 
 
@@ -154,14 +169,23 @@ def write_spectrum(root, p, t, nharm, eta, idf):
 def read_dens(path, single=False):
     """Returns (psi, t_plus_im_t, (nyprocs, nzprocs), (js, je, ks, ke), x, y, z)."""
     rt, ct = (np.float32, np.complex64) if single else (np.float64, np.complex128)
+    rs, cs = np.dtype(rt).itemsize, np.dtype(ct).itemsize
+    size = os.path.getsize(path)
     with open(path, "rb") as fh:
-        tt = float(np.frombuffer(fh.read(np.dtype(rt).itemsize), dtype=rt)[0])
-        nx, ny, nz, nyp, nzp, js, je, ks, ke = (int(v) for v in np.frombuffer(fh.read(36), dtype=np.int32))
+        tt = float(np.frombuffer(_read_exact(fh, rs, "time", path), dtype=rt)[0])
+        nx, ny, nz, nyp, nzp, js, je, ks, ke = (int(v) for v in np.frombuffer(_read_exact(fh, 36, "header", path),
+                                                                                dtype=np.int32))
+        if min(nx, ny, nz) < 1 or not (0 <= js <= je < ny and 0 <= ks <= ke < nz):
+            raise ValueError("%s: grid %dx%dx%d with rows %d..%d, planes %d..%d in the header" %
+                             (path, nx, ny, nz, js, je, ks, ke))
         n = nx * (je - js + 1) * (ke - ks + 1)
-        psi = np.frombuffer(fh.read(n * np.dtype(ct).itemsize), dtype=ct).reshape(ke - ks + 1, je - js + 1, nx)
-        x = np.frombuffer(fh.read(nx * np.dtype(rt).itemsize), dtype=rt)
-        y = np.frombuffer(fh.read(ny * np.dtype(rt).itemsize), dtype=rt)
-        z = np.frombuffer(fh.read(nz * np.dtype(rt).itemsize), dtype=rt)
+        want = rs + 36 + n * cs + (nx + ny + nz) * rs
+        if size != want:
+            raise ValueError("%s: %d bytes, the header describes %d" % (path, size, want))
+        psi = np.frombuffer(_read_exact(fh, n * cs, "field", path), dtype=ct).reshape(ke - ks + 1, je - js + 1, nx)
+        x = np.frombuffer(_read_exact(fh, nx * rs, "x", path), dtype=rt)
+        y = np.frombuffer(_read_exact(fh, ny * rs, "y", path), dtype=rt)
+        z = np.frombuffer(_read_exact(fh, nz * rs, "z", path), dtype=rt)
     return psi.copy(), tt, (nyp, nzp), (js, je, ks, ke), x.copy(), y.copy(), z.copy()
 
 

@@ -362,6 +362,40 @@ int gpe_batch_tune_get(gpe_batch *b, int32_t precision, int32_t order, int32_t s
  * that beats its two stages).  Commits no step. */
 int gpe_batch_autotune(gpe_batch *b, int32_t reps);
 
+/* ---- Serving: continuous batching behind a request queue ---------------
+ *
+ * No reference counterpart (the reference runs one simulation per job).
+ * Requests -- a solver() loop of nsteps steps from a state and a field --
+ * are queued in submission order.  Each gpe_serve_step admits queued
+ * requests first come first served while their device footprint fits the
+ * HBM budget and the resident grid points stay within max_sites (the first
+ * request that does not fit waits, and the ones behind it), runs
+ * `iterations` batch iterations (every resident request advances one step,
+ * rk45 one attempt; requests admitted now join the running iteration), and
+ * retires finished requests: field and state are kept in host memory until
+ * gpe_serve_result, and their device memory returns to the pool.  A
+ * request's result equals the same request run alone in a batch. */
+typedef struct gpe_serve gpe_serve;
+/* hbm_budget: device bytes for resident requests (0 = 90% of free memory);
+ * max_sites: cap on resident grid points (0 = none). */
+int gpe_serve_create(int64_t hbm_budget, int64_t max_sites, gpe_serve **out);
+void gpe_serve_destroy(gpe_serve *s);
+/* Queue a request.  psi: the initial field (nx*ny*nz of host_type), copied;
+ * or NULL for the random-phase state of gpe_sim_random_phase(seed, kmax),
+ * built on the device at admission.  GPE_EFULL if the request can never fit
+ * the budget.  *ticket names the request. */
+int gpe_serve_submit(gpe_serve *s, const gpe_params *params, const void *psi, int32_t host_type, uint64_t seed,
+                     double kmax, const gpe_step_state *st, int64_t nsteps, int64_t *ticket);
+/* One scheduling round (admit, run `iterations`, retire); the counts of
+ * resident and queued requests after it.  Synchronous. */
+int gpe_serve_step(gpe_serve *s, int32_t iterations, int32_t *running, int32_t *queued);
+/* Requests finished since the last poll: tickets and status (1 done, -1 NaN
+ * in the rk45 error norm, -2 more than 1000 rejections); n <= cap. */
+int gpe_serve_poll(gpe_serve *s, int64_t *tickets, int32_t *status, int32_t cap, int32_t *n);
+/* Field (host_type) and state of a finished request; releases it
+ * (GPE_ESTATE if it has not finished or was collected). */
+int gpe_serve_result(gpe_serve *s, int64_t ticket, void *psi, int32_t host_type, gpe_step_state *st);
+
 #ifdef __cplusplus
 }
 #endif
