@@ -58,7 +58,7 @@ EXPORTS = [
     "gpe_batch_create", "gpe_batch_destroy", "gpe_batch_footprint", "gpe_batch_add",
     "gpe_batch_remove", "gpe_batch_run", "gpe_batch_sync", "gpe_batch_poll", "gpe_batch_query",
     "gpe_batch_download", "gpe_batch_stats", "gpe_batch_use_graph", "gpe_batch_tune_set",
-    "gpe_batch_tune_get", "gpe_batch_autotune",
+    "gpe_batch_tune_get", "gpe_batch_autotune", "gpe_energy_noalloc",
     "gpe_serve_create", "gpe_serve_destroy", "gpe_serve_submit", "gpe_serve_step", "gpe_serve_poll",
     "gpe_serve_result",
 ]
@@ -165,6 +165,7 @@ def lib():
         "gpe_batch_tune_set": ([vp, i32, i32, i32, i32, i32, i32], ctypes.c_int),
         "gpe_batch_tune_get": ([vp, i32, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)], ctypes.c_int),
         "gpe_batch_autotune": ([vp, i32], ctypes.c_int),
+        "gpe_energy_noalloc": ([P, vp, dp, vp], ctypes.c_int),
         "gpe_serve_create": ([i64, i64, ctypes.POINTER(vp)], ctypes.c_int),
         "gpe_serve_destroy": ([vp], None),
         "gpe_serve_submit": ([vp, P, vp, i32, ctypes.c_uint64, ctypes.c_double, S, i64, ctypes.POINTER(i64)],

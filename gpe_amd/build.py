@@ -31,6 +31,7 @@ UNITS = [("gpe_misc.hip", "gpe_misc", ["-ffp-contract=off"]),
          ("gpe_ic.cpp", "gpe_ic", ["-ffp-contract=off", "-x", "hip"]),
          ("tune_table.cpp", "tune_table", ["-x", "hip"]),
          ("gpe_spectral.cpp", "gpe_spectral", ["-ffp-contract=off", "-x", "hip"]),
+         ("gpe_noalloc.cpp", "gpe_noalloc", ["-ffp-contract=off", "-x", "hip"]),
          ("gpe_serve.cpp", "gpe_serve", ["-x", "hip"]),
          ("errors.cpp", "errors", ["-x", "hip"]),
          ("ranges.cpp", "ranges", ["-x", "hip"])]

@@ -244,6 +244,26 @@ int gpe_sim_renormalise(gpe_ctx *ctx, double nn, double norm);
 /* f(psi) of the resident field into a host buffer (get_rhs, solve.f90:382). */
 int gpe_sim_rhs(gpe_ctx *ctx, void *out, int32_t host_type);
 
+/* energy() (variables.f90:580-604, as gpe_sim_energy) of a field the caller
+ * holds in device memory, allocating nothing.  No reference counterpart:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).
+ *   params  grid, spacing, order, bcs and precision of the field (the
+ *           scheme and step-control fields are not read)
+ *   psi     device field, nx*ny*nz complex128 (GPE_FP64) or complex64
+ *           (GPE_FP32, and the high word of GPE_FP32C)
+ *   energy  device pointer to one double, written on `stream`
+ *   stream  a hipStream_t, or NULL for the null stream
+ * Asynchronous: the result is in *energy when the stream reaches it.  The
+ * per-workgroup partial sums live in a static device array of 65,536
+ * doubles per GPU (512 KiB, held until the process ends); grids that need
+ * more workgroups than that (with the default (8,2) tile and automatic
+ * z-chunks: far beyond one GPU's memory) return GPE_EINVAL.  Calls that
+ * overlap on one GPU (different streams or threads) share that array and
+ * race; calls on one stream are ordered.  The result equals gpe_sim_energy's
+ * on a context with the default launch configuration, bit for bit. */
+int gpe_energy_noalloc(const gpe_params *params, const void *psi, double *energy, void *stream);
+
 /* Launch configuration of the stage kernels.  Each stage kind (0 .. 14)
  * and each fused stage pair (16 .. 20; see gpe_prof_query for names) runs
  * with one of several compiled tile variants (waves x rows per wave) and a
