@@ -269,7 +269,11 @@ def serve_leg(rate, duration, seed, seq_budget_s=30.0, log_=log):
     from gpe_amd.serve import Server
     from gpe_amd.workloads import request_stream
     reqs = request_stream(rate, duration, seed)
-    with Server(max_sites=64e6) as s:
+    # launch configuration of each (precision, order, scheme) group tuned on
+    # the resident requests as groups appear (at most every 3 s; no step is
+    # committed by tuning); requests that fit the resident-site cap may pass
+    # one that does not until it has waited 5 s
+    with Server(max_sites=64e6, autotune_every_s=3.0, backfill_s=5.0) as s:
         m = s.serve(reqs, duration=duration, drain=False, log=lambda *x: log_("serve", *x))
         done = [d[0] for d in s.last_done if d[4] == 1]
         model_bytes = s.last_model_bytes

@@ -145,7 +145,20 @@ int gpe_serve_step(gpe_serve *s, int32_t iterations, int32_t *running, int32_t *
     else
       rc = gpe_batch_add(s->batch, &r.P, r.field.data(), r.host_type, &r.st, r.nsteps, &id);
     if (rc == GPE_EFULL) break;  // waits for memory
-    if (rc) return rc;
+    if (rc) {
+      // cannot be admitted at all (e.g. a device error building its state):
+      // it leaves the queue as finished with status -3, so the requests
+      // behind it are not held up, and the call reports the error
+      Result res;
+      res.status = -3;
+      res.P = r.P;
+      res.st = r.st;
+      const std::string msg = gpe::g_err;
+      s->unreported.emplace_back(r.ticket, res.status);
+      s->finished.emplace(r.ticket, std::move(res));
+      s->queue.pop_front();
+      return fail(rc, "request %lld not admitted: %s", (long long)r.ticket, msg.c_str());
+    }
     s->resident_sites += r.sites;
     r.field.clear();
     r.field.shrink_to_fit();
@@ -203,6 +216,11 @@ int gpe_serve_result(gpe_serve *s, int64_t ticket, void *psi, int32_t host_type,
   auto it = s->finished.find(ticket);
   if (it == s->finished.end()) return fail(GPE_ESTATE, "request %lld has not finished (or was collected)", (long long)ticket);
   const Result &r = it->second;
+  if (r.field.empty()) {
+    const int status = r.status;
+    s->finished.erase(it);
+    return fail(GPE_ESTATE, "request %lld was not admitted (status %d): no field", (long long)ticket, status);
+  }
   const size_t n = sites_of(r.P);
   const double *src = reinterpret_cast<const double *>(r.field.data());
   if (host_type == GPE_HOST_C128) {

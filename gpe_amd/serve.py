@@ -55,14 +55,24 @@ class _Ready:
 
 
 class Server:
-    def __init__(self, hbm_budget=0, max_sites=None, target_chunk_s=0.05, ic_workers=4, prefetch=8,
-                 deliver=None, device_ic=True):
+    def __init__(self, hbm_budget=0, max_sites=None, target_chunk_s=0.15, ic_workers=4, prefetch=8,
+                 deliver=None, device_ic=True, autotune_every_s=None, backfill_s=None):
         """hbm_budget: device bytes for resident simulations (0: 90% of free).
         max_sites: cap on resident grid points (None: no cap).
+        target_chunk_s: device time per chunk of iterations between
+        retirements and admissions.
         deliver(request, psi, info): called with each finished result.
         device_ic: build random-phase initial states on the device (hipFFT)
         instead of on the host (the other states are built on the host by
-        ic_workers threads, `prefetch` requests ahead)."""
+        ic_workers threads, `prefetch` requests ahead).
+        autotune_every_s: when requests of a (precision, order, scheme)
+        group not yet tuned are resident, and at least this many seconds
+        after the last tuning, time the tile variants and z-chunks of every
+        group on the resident instances (gpe_batch_autotune: no step is
+        committed); None: keep the default launch configuration.
+        backfill_s: let queued requests that fit the resident-site cap go
+        ahead of one that does not, until the queue's head has waited this
+        long (None: strict first come first served)."""
         self.batch = Batch(hbm_budget)
         self.budget = self.batch.stats()["budget"]
         self.max_sites = max_sites
@@ -71,6 +81,8 @@ class Server:
         self.prefetch = prefetch
         self.deliver = deliver
         self.device_ic = device_ic
+        self.autotune_every_s = autotune_every_s
+        self.backfill_s = backfill_s
 
     def close(self):
         self.pool.shutdown(wait=True)
@@ -113,6 +125,7 @@ class Server:
                 resident_sites += r.cfg.nx * r.cfg.ny * r.cfg.nz
         done = []         # (request, arrival, admitted, finished, status, steps)
         chunk = 4
+        tuned, last_tune = set(), -1e30
         busy_s = 0.0
         su_done = 0.0
         model_bytes = 0.0
@@ -134,12 +147,21 @@ class Server:
             while pending and pending[0].arrival <= now and (duration is None or pending[0].arrival < duration):
                 queue.append(pending.popleft())
             prefetch()
-            # admission, first come first served
-            while queue:
-                r = queue[0]
+            # admission, first come first served.  With backfill_s set, a
+            # request that does not fit the resident-site cap lets the ones
+            # behind it that do fit go ahead, until the queue's head has
+            # waited backfill_s; then nothing passes it.  The HBM budget
+            # (memory) stops admission outright.
+            i = 0
+            while i < len(queue):
+                r = queue[i]
                 sites = r.cfg.nx * r.cfg.ny * r.cfg.nz
                 if self.max_sites is not None and running and resident_sites + sites > self.max_sites:
-                    break
+                    head_waited = clock() - queue[0].arrival
+                    if self.backfill_s is None or head_waited >= self.backfill_s:
+                        break
+                    i += 1
+                    continue
                 if _batch.footprint(r.cfg) > self.budget:
                     raise ValueError("request %d can never fit the HBM budget" % r.rid)
                 if on_device(r):
@@ -157,7 +179,7 @@ class Server:
                     if iid is None:  # does not fit now: wait for memory
                         futures[r.rid] = _Ready(psi)
                         break
-                queue.popleft()
+                del queue[i]
                 running[iid] = (r, clock(), sites)
                 resident_sites += sites
                 prefetch()
@@ -173,6 +195,12 @@ class Server:
                 continue
             if duration is not None and not drain and clock() >= duration:
                 break
+            if self.autotune_every_s is not None:
+                keys = {(r.cfg.precision, r.cfg.order, r.cfg.scheme) for r, _, _ in running.values()}
+                if keys - tuned and clock() - last_tune >= self.autotune_every_s:
+                    self.batch.autotune(reps=1)
+                    tuned |= keys
+                    last_tune = clock()
             ts = time.perf_counter()
             self.batch.run(chunk)
             finished = self.batch.poll()
@@ -338,9 +366,17 @@ def main():
     ap.add_argument("--max-msites", type=float, default=64.0, help="resident grid points cap, millions")
     ap.add_argument("--no-drain", action="store_true")
     ap.add_argument("--host-ic", action="store_true", help="build random-phase states on the host")
+    ap.add_argument("--chunk-s", type=float, default=0.15, help="device time per chunk of iterations, seconds")
+    ap.add_argument("--autotune-every-s", type=float, default=3.0,
+                    help="tune the launch configuration of new groups, at most this often (seconds; <= 0: never)")
+    ap.add_argument("--backfill-s", type=float, default=None,
+                    help="let smaller requests pass one that does not fit, until it has waited this long (seconds)")
     a = ap.parse_args()
     reqs = request_stream(a.rate, a.duration, a.seed)
-    with Server(hbm_budget=int(a.budget_gib * 2 ** 30), max_sites=a.max_msites * 1e6, device_ic=not a.host_ic) as s:
+    with Server(hbm_budget=int(a.budget_gib * 2 ** 30), max_sites=a.max_msites * 1e6, device_ic=not a.host_ic,
+                target_chunk_s=a.chunk_s,
+                autotune_every_s=a.autotune_every_s if a.autotune_every_s > 0 else None,
+                backfill_s=a.backfill_s) as s:
         m = s.serve(reqs, duration=a.duration, drain=not a.no_drain,
                     log=lambda *x: print("[serve]", *x, file=sys.stderr, flush=True))
     m.update({"rate": a.rate, "duration": a.duration, "requests": len(reqs)})
