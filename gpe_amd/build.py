@@ -5,7 +5,15 @@
 Each translation unit is compiled with hipcc; the FP64 one with
 -ffp-contract=off so its arithmetic is one rounding per operation in the
 reference's order.  Packed FP32 instructions are disabled on every unit.
+
+Every build recompiles every unit, holds a lock on its object folder (two
+builds never interleave their objects) and refuses to link when a file under
+csrc/ changed while it compiled: the host code and the kernels share
+argument structs by layout, and objects compiled from two versions of a
+header would link without complaint.
 """
+import fcntl
+import hashlib
 import os
 import subprocess
 import sys
@@ -61,6 +69,17 @@ def _compile(unit):
     return out
 
 
+def _sources_digest():
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(CSRC)):
+        path = os.path.join(CSRC, name)
+        if os.path.isfile(path):
+            h.update(name.encode() + b"\0")
+            with open(path, "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()
+
+
 def build(verbose=True, extra_flags=(), lib_path=None, obj_dir=None):
     """Compile every unit and link the library.  extra_flags/lib_path/obj_dir
     build an alternative library (e.g. -DGPE_NT_STORE) beside the default."""
@@ -71,10 +90,20 @@ def build(verbose=True, extra_flags=(), lib_path=None, obj_dir=None):
     if extra_flags:
         global UNITS
         UNITS = [(a, b, c + list(extra_flags)) for a, b, c in UNITS]
-    with ThreadPoolExecutor(max_workers=min(len(UNITS), os.cpu_count() or 4)) as ex:
-        objs = list(ex.map(_compile, UNITS))
+    with open(os.path.join(OBJ, ".lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        before = _sources_digest()
+        with ThreadPoolExecutor(max_workers=min(len(UNITS), os.cpu_count() or 4)) as ex:
+            objs = list(ex.map(_compile, UNITS))
+        if _sources_digest() != before:
+            raise RuntimeError("files under %s changed during the build: its objects may mix two versions of "
+                               "the shared headers; build again" % CSRC)
+        return _link(objs, verbose, lib_path)
+
+
+def _link(objs, verbose, lib_path):
     out = lib_path or LIB
-    tmp = out + ".tmp"
+    tmp = "%s.tmp%d" % (out, os.getpid())
     # hipFFT (the random-phase initial state) from the ROCm install
     rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "lib")
     cmd = ([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
