@@ -241,6 +241,55 @@ def test_fp32_forms_track_fp64_rk45(gpu, oracle, precision):
     assert same >= 8
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+def test_fp32_forms_at_fp64_steps(gpu, oracle, precision):
+    # every one of the oracle's accepted steps, taken by the FP32 form from
+    # the oracle's field with the oracle's accepted dt (tolerance 1e20: the
+    # first attempt is accepted): psi within 1e-6 of the oracle's on all 12,
+    # and the unscaled error norm max|err/scale| close to the oracle's (its
+    # k fields carry single-precision rounding into err, a difference of
+    # fifth- and fourth-order solutions ~1e-6 of the scale)
+    from gpe_amd.sim import Simulation
+    cfg64 = small_cfg(scheme="rk45", order=4, bcs=1, tau=0.01)
+    cfg = cfg64.with_(precision=precision, eps=1e20)
+    psi = ring_ic(cfg64)
+    say("fp32 form at fp64 steps", precision)
+    # plain FP32 runs its step control (t, dt) in single precision
+    t_rel = 1e-6 if precision == "fp32" else 1e-14
+    st = oracle.make_state(cfg64.dt)
+    cur = psi
+    worst_psi = worst_err = 0.0
+    rejected = compared = 0
+    with Simulation(cfg, psi) as sim:
+        for i in range(12):
+            t0, start = st.t, cur
+            cur = oracle.solver(cfg64, st, cur, NT)
+            h = st.t - t0
+            rejected += st.n_rejects
+            sim.set_state(h, t=t0)
+            sim.upload(start)
+            sim.advance(1)
+            s = sim.state()
+            assert s["n_rejects"] == 0
+            assert s["t"] == pytest.approx(st.t, rel=t_rel)
+            got = sim.download(np.complex128)
+            e_psi = rel_l2(got, cur)
+            worst_psi = max(worst_psi, e_psi)
+            e_norm = abs(s["errmax"] * cfg.eps / (st.errmax * cfg64.eps) - 1.0)
+            say("step %d dt %.6e rejections %d: psi rel L2 %.3e, error norm rel diff %.3e"
+                % (i, h, st.n_rejects, e_psi, e_norm))
+            # after a rejection the oracle's accepted attempt still uses the
+            # error scale formed with the first attempt's dt (solve.f90:176-178),
+            # the forced single attempt its own: compare steps without one
+            if st.n_rejects == 0:
+                worst_err = max(worst_err, e_norm)
+                compared += 1
+    say("worst psi %.3e, worst error norm %.3e over %d steps, oracle rejections %d"
+        % (worst_psi, worst_err, compared, rejected))
+    assert worst_psi < 1e-6
+    assert compared >= 8 and worst_err < 0.05
+
+
 def test_ring_64cube_rk4_1000_steps(gpu, oracle):
     # 64^3, dx = 0.4, reflective, 2nd order, RK4 dt = 0.01, 1000 steps, V = 0
     cfg = RunConfig(nx=64, ny=64, nz=64, xr=xr_for_spacing(0.4, 64), bcs=2, order=2, scheme="rk4",
@@ -373,10 +422,15 @@ def test_plane_wave_512_rk45_matches_scalar_recursion(gpu, precision):
         assert ts[-1] == pytest.approx(t, rel=1e-6)
         assert err < 1e-9
     else:
-        # FP32 storage: the error norm is a max over 1.3e8 sites, and the
-        # stencil of psi's FP32 rounding (6e-8 relative, times ~1/dx^2) puts
-        # the worst site's noise near the tolerance at these huge steps, so
-        # the compensated form may choose smaller steps than FP64; psi still
-        # follows the exact recursion for the steps it took
+        # FP32 storage: the steps the exact control takes (0.05, 0.25, 0.6)
+        # lie far outside the explicit scheme's stability region for the
+        # lattice's shortest waves: the 4th-order stencil's largest symbol is
+        # 16/(3 dx^2) per axis, so |z| = dt * 0.5 * 3 * 16/(3 dx^2) = 32 dt,
+        # about 19 at dt = 0.6.  The plane wave has no content there in FP64
+        # (rounding ~1e-16 grows to ~1e-10, the FP64 case above), but FP32
+        # rounding (~6e-8) grows to ~5e-3 over those three steps (measured
+        # with the steps forced on the compensated form: 5.4e-3).  The error
+        # norm sees that growth, so the compensated form takes smaller steps,
+        # as it should; psi follows the exact recursion for the steps it took
         assert 0.2 * t < ts[-1] <= t * (1 + 1e-6)
         assert err < 1e-5
