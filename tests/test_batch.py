@@ -272,3 +272,50 @@ def test_batch_fused_pairs_equal_unfused(precision, scheme, variant):
                 assert b.tune_get(precision, 2, scheme, pairs[0]) == (variant, 16)
     for a, c in zip(res[0], res[1]):
         assert np.array_equal(a, c)
+
+
+# Instances at the edges of the tile geometry: 5 points per axis, smaller
+# than one tile in every direction, one point past a tile multiple, z shorter
+# than the prefetch window, beside an ordinary grid.
+EDGE = [
+    dict(n=(5, 5, 5), bcs=2, order=4, om=0.30),
+    dict(n=(6, 40, 5), bcs=1, order=2, om=0.20),
+    dict(n=(65, 17, 9), bcs=2, order=2, om=0.15),
+    dict(n=(5, 7, 64), bcs=1, order=4, om=0.25),
+    dict(n=(32, 32, 32), bcs=2, order=4, om=0.12),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_batch_edge_shapes_rk4_equal_single(precision):
+    entries = []
+    for k, m in enumerate(EDGE):
+        c = cfg_of(m["n"], order=m["order"], bcs=m["bcs"], precision=precision, om=m["om"], tau=0.005)
+        entries.append((c, ring(c), 5 + 2 * k))
+    say("batch edge shapes rk4 vs single", precision, [e[0].shape for e in entries])
+    got = _batch(entries, chunk=4)
+    for (c, p, n), (g, q) in zip(entries, got):
+        want, st, nrm = _single(c, p, n)
+        assert q["steps"] == n and q["status"] == 1
+        assert np.array_equal(g, want), (c.shape, rel_l2(g, want))
+        assert q["t"] == st["t"]
+        assert abs(q["norm"] - nrm) <= 1e-12 * abs(nrm)
+
+
+@pytest.mark.gpu
+def test_batch_edge_shapes_fp64_rk45_vs_oracle(oracle):
+    # the batch's device step control rounds pow() correctly; so does the
+    # oracle in this mode, so every instance matches it bit for bit
+    entries = []
+    for m in EDGE:
+        c = cfg_of(m["n"], scheme="rk45", order=m["order"], bcs=m["bcs"], om=m["om"], tau=0.05)
+        entries.append((c, ring(c), 6))
+    say("batch edge shapes fp64 rk45 vs oracle", [e[0].shape for e in entries])
+    got = _batch(entries, chunk=4)
+    with oracle.correctly_rounded_control_pow():
+        for (c, p, n), (g, q) in zip(entries, got):
+            st = oracle.make_state(c.dt)
+            want, _ = oracle.run(c, st, p, n, NT)
+            assert np.array_equal(g, want), (c.shape, rel_l2(g, want))
+            assert q["t"] == st.t and q["dt"] == st.dt_re
