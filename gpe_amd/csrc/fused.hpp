@@ -165,7 +165,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
     } else {
       cx<T> u = r.ps;
 #pragma unroll
-      for (int m = 0; m < NIN; ++m) u = axpy<COMP>((T)A.b[m], r.kk[m], u);
+      for (int m = 0; m < NIN; ++m) u = axpy<COMP>((T)ck_b(KA - K_CK_1 + 1, m), r.kk[m], u);
       return u;
     }
   };
@@ -176,7 +176,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
     } else {
       cx<T> u = r.ps;
 #pragma unroll
-      for (int m = 0; m < NIN; ++m) u = axpy<COMP>((T)B.b[m], r.kk[m], u);
+      for (int m = 0; m < NIN; ++m) u = axpy<COMP>((T)ck_b(KB - K_CK_1 + 1, m), r.kk[m], u);
       return u;
     }
   };
@@ -184,7 +184,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
     if constexpr (KindInfo<KB>::complex_coef)
       return w + cmul(HB, ka);
     else
-      return axpy<COMP>((T)B.b[NIN], ka, w);
+      return axpy<COMP>((T)ck_b(KB - K_CK_1 + 1, NIN), ka, w);
   };
   // B's final-combination partials known at load time
   auto make_F = [&](const Raw &r, cx<T> *F) {
@@ -192,13 +192,13 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
       F[0] = r.ps;
       F[1] = r.ac;
     } else if constexpr (KB == K_CK_6) {
-      cx<T> e = rmul((T)B.dc[1], r.kk[0]);
+      cx<T> e = rmul((T)ck_dc(1), r.kk[0]);
 #pragma unroll
-      for (int m = 1; m < 4; ++m) e = axpy<COMP>((T)B.dc[m + 1], r.kk[m], e);
+      for (int m = 1; m < 4; ++m) e = axpy<COMP>((T)ck_dc(m + 1), r.kk[m], e);
       if constexpr (COMP) {
-        cx<T> pd = rmul((T)B.c[1], r.kk[0]);  // c2 = c5 = 0
-        pd = axpy<COMP>((T)B.c[3], r.kk[2], pd);
-        pd = axpy<COMP>((T)B.c[4], r.kk[3], pd);
+        cx<T> pd = rmul((T)ck_c(1), r.kk[0]);  // c2 = c5 = 0
+        pd = axpy<COMP>((T)ck_c(3), r.kk[2], pd);
+        pd = axpy<COMP>((T)ck_c(4), r.kk[3], pd);
         F[0] = r.ps;
         F[1] = pd;
         F[2] = e;
@@ -206,7 +206,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
       } else {
         cx<T> s = r.ps;
 #pragma unroll
-        for (int m = 0; m < 4; ++m) s = s + rmul((T)B.c[m + 1], r.kk[m]);
+        for (int m = 0; m < 4; ++m) s = s + rmul((T)ck_c(m + 1), r.kk[m]);
         F[0] = s;
         F[1] = e;
       }
@@ -220,10 +220,10 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
       F[1] = F[1] + rdiv(ka, B.d3, 1.0 / 3.0);
     } else if constexpr (KB == K_CK_6) {
       if constexpr (COMP) {
-        F[2] = axpy<COMP>((T)B.dc[5], ka, F[2]);
+        F[2] = axpy<COMP>((T)ck_dc(5), ka, F[2]);
       } else {
-        F[0] = F[0] + rmul((T)B.c[5], ka);
-        F[1] = F[1] + rmul((T)B.dc[5], ka);
+        F[0] = F[0] + rmul((T)ck_c(5), ka);
+        F[1] = F[1] + rmul((T)ck_dc(5), ka);
       }
     }
   };
@@ -503,16 +503,16 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
             stc<T>(B.kout, s, kk);
           } else {
             const cx<T> sc = COMP ? F[NF - 1] : oc[p][0];
-            const cx<T> err = F[COMP ? 2 : 1] + rmul((T)B.dc[6], kk);
+            const cx<T> err = F[COMP ? 2 : 1] + rmul((T)ck_dc(6), kk);
             cx<double> np;
             if constexpr (IS_DBL) {
-              np = F[0] + rmul((T)B.c[6], kk);
+              np = F[0] + rmul((T)ck_c(6), kk);
               stc<T>(B.psi_out, s, np);
               const double e = cabs_(cdiv(err, sc));
               acc_emax = (e > acc_emax || e != e) ? e : acc_emax;
             } else {
               if constexpr (COMP) {
-                const cx<double> dl = todbl(F[1]) + rmul(B.c[6], todbl(kk));
+                const cx<double> dl = todbl(F[1]) + rmul(ck_c(6), todbl(kk));
                 np = (todbl(F[0]) + todbl(oc[p][0])) + dl;
                 const cx<T> hi = mk((T)np.re, (T)np.im);
                 stc<T>(B.psi_out, s, hi);
@@ -520,7 +520,7 @@ __device__ __forceinline__ void pair_tile(const StageArgs &A, const StageArgs &B
                 const double e2 = err_ratio2(err, sc);  // |err/scale|^2, as the one-stage kernel
                 acc_emax = (e2 > acc_emax || e2 != e2) ? e2 : acc_emax;
               } else {
-                const cx<T> nf = F[0] + rmul((T)B.c[6], kk);
+                const cx<T> nf = F[0] + rmul((T)ck_c(6), kk);
                 stc<T>(B.psi_out, s, nf);
                 np = todbl(nf);
                 const float e = cabs_(cdiv(err, sc));  // abs(err/scale) (solve.f90:184)

@@ -283,14 +283,6 @@ StageArgs inst_args(const Inst &I, int kind, const gpe_step_state &st) {
   A.psi_out = I.buf[1];
   A.lo = I.lob[0];
   A.lo_out = I.lob[1];
-  static const double bt[7][5] = {{0, 0, 0, 0, 0},
-                                  {0, 0, 0, 0, 0},
-                                  {0.2, 0, 0, 0, 0},
-                                  {0.075, 0.225, 0, 0, 0},
-                                  {0.3, -0.9, 1.2, 0, 0},
-                                  {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0},
-                                  {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0,
-                                   253.0 / 4096.0}};
   switch (kind) {
     case K_RK2_1: A.kout = I.k[0]; break;
     case K_RK2_2: A.kin[0] = I.k[0]; break;
@@ -313,7 +305,6 @@ StageArgs inst_args(const Inst &I, int kind, const gpe_step_state &st) {
       if (kind >= K_CK_1 && kind <= K_CK_6) {
         const int s = kind - K_CK_1 + 1;
         for (int m = 0; m < s - 1; ++m) A.kin[m] = I.k[m];
-        for (int m = 0; m < 5; ++m) A.b[m] = bt[s][m];
         if (s < 6) A.kout = I.k[s - 1];
         if (s == 1) A.scale_out = I.scale;
         if (s == 6) A.scale_in = I.scale;

@@ -390,15 +390,6 @@ int rkqs(gpe_ctx *c, double *hist_slot) {
       A.psi = c->psi;
       A.lo = c->lo;
       for (int m = 0; m < s - 1; ++m) A.kin[m] = c->k[m];
-      static const double bt[7][5] = {{0, 0, 0, 0, 0},
-                                      {0, 0, 0, 0, 0},
-                                      {0.2, 0, 0, 0, 0},
-                                      {0.075, 0.225, 0, 0, 0},
-                                      {0.3, -0.9, 1.2, 0, 0},
-                                      {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0},
-                                      {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0,
-                                       253.0 / 4096.0}};
-      for (int m = 0; m < 5; ++m) A.b[m] = bt[s][m];
       if (s < 6) {
         A.kout = c->k[s - 1];
         if (s == 1) {
@@ -1288,10 +1279,7 @@ int gpe_autotune(gpe_ctx *c, int32_t reps) {
       X->psi = c->psi;
       X->lo = c->lo;
       X->write_scale = 0;
-      for (int m = 0; m < 5; ++m) {
-        X->kin[m] = c->k[m] ? c->k[m] : c->psi;
-        X->b[m] = 0.1;
-      }
+      for (int m = 0; m < 5; ++m) X->kin[m] = c->k[m] ? c->k[m] : c->psi;
       X->psi_out = c->psi2;
       X->lo_out = c->lo2;
       X->acc_in = c->acc;
@@ -1367,7 +1355,6 @@ int gpe_autotune(gpe_ctx *c, int32_t reps) {
     A.kout = c->k[nin < 5 ? nin : 0];
     if (kind == K_RK4_2) A.kout = c->k[1];
     if (kind == K_RK4_3) { A.kin[0] = c->k[1]; A.kout = c->k[0]; }
-    for (int m = 0; m < 5; ++m) A.b[m] = 0.1;
     std::vector<int> zcs = {0, 16, 32, 64, 128, 256, 512};  // as for the pairs
     if (c->P.nz > 512) zcs.push_back(c->P.nz);
     float best = 1e30f;

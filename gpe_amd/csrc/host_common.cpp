@@ -79,22 +79,6 @@ void fill_base(const gpe_params *params, int prec, StageArgs &A) {
   A.d_x = make_divc(sx);
   A.d6 = make_divc(6.0);
   A.d3 = make_divc(3.0);
-  // Cash-Karp weights (solve.f90:270-281), formed in double as the
-  // reference forms its real(pr) parameters
-  const double c1 = 37.0 / 378.0, c2 = 0.0, c3 = 250.0 / 621.0, c4 = 125.0 / 594.0, c5 = 0.0,
-               c6 = 512.0 / 1771.0;
-  A.c[1] = c1;
-  A.c[2] = c2;
-  A.c[3] = c3;
-  A.c[4] = c4;
-  A.c[5] = c5;
-  A.c[6] = c6;
-  A.dc[1] = c1 - 2825.0 / 27648.0;
-  A.dc[2] = c2 - 0.0;
-  A.dc[3] = c3 - 18575.0 / 48384.0;
-  A.dc[4] = c4 - 13525.0 / 55296.0;
-  A.dc[5] = c5 - 277.0 / 14336.0;
-  A.dc[6] = c6 - 0.25;
   A.w1 = 3.0 / 8.0;
   A.w2 = 7.0 / 6.0;
   A.w3 = 23.0 / 24.0;

@@ -1,6 +1,8 @@
 // stage_args.h -- POD launch descriptor shared by the host code and the
 // stage kernels (passed by value as the kernel argument).
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstddef>
 #include <cstdint>
 
@@ -24,6 +26,38 @@ enum Kind : int {
   K_ENERGY,     // sum Re(-lap conj psi) + 0.5|psi|^4  (variables.f90:580-604)
   K_COUNT
 };
+
+// Cash-Karp tableau (solve.f90:255-281), each weight formed in double as
+// the reference forms its real(pr) parameters.  The kernels take them as
+// compile-time constants: no scalar registers hold them across the z march.
+// ck_b(s, m): coefficient of kk_{m+1} in the input of stage s (2..6).
+__host__ __device__ constexpr double ck_b(int s, int m) {
+  return s == 2   ? (m == 0 ? 0.2 : 0.0)
+         : s == 3 ? (m == 0 ? 0.075 : m == 1 ? 0.225 : 0.0)
+         : s == 4 ? (m == 0 ? 0.3 : m == 1 ? -0.9 : m == 2 ? 1.2 : 0.0)
+         : s == 5 ? (m == 0 ? -11.0 / 54.0 : m == 1 ? 2.5 : m == 2 ? -70.0 / 27.0 : m == 3 ? 35.0 / 27.0 : 0.0)
+         : s == 6 ? (m == 0   ? 1631.0 / 55296.0
+                     : m == 1 ? 175.0 / 512.0
+                     : m == 2 ? 575.0 / 13824.0
+                     : m == 3 ? 44275.0 / 110592.0
+                     : m == 4 ? 253.0 / 4096.0
+                              : 0.0)
+                  : 0.0;
+}
+// ck_c(m): weight of kk_m in the fifth-order solution (m = 1..6).
+__host__ __device__ constexpr double ck_c(int m) {
+  return m == 1 ? 37.0 / 378.0 : m == 3 ? 250.0 / 621.0 : m == 4 ? 125.0 / 594.0 : m == 6 ? 512.0 / 1771.0 : 0.0;
+}
+// ck_dc(m): c_m minus the embedded fourth-order weight (the error estimate).
+__host__ __device__ constexpr double ck_dc(int m) {
+  return m == 1   ? ck_c(1) - 2825.0 / 27648.0
+         : m == 2 ? ck_c(2) - 0.0
+         : m == 3 ? ck_c(3) - 18575.0 / 48384.0
+         : m == 4 ? ck_c(4) - 13525.0 / 55296.0
+         : m == 5 ? ck_c(5) - 277.0 / 14336.0
+         : m == 6 ? ck_c(6) - 0.25
+                  : 0.0;
+}
 
 // Two consecutive stages fused into one sweep (fused.hpp).
 enum Pair : int { P_RK4_12 = 0, P_RK4_34, P_CK_12, P_CK_34, P_CK_56, P_COUNT };
@@ -63,8 +97,6 @@ struct StageArgs {
   double hr, hi;     // 0.5*dt
   double sdtr, sdti; // dt at entry of the rk45 step (scale)
   double inv_absdt;  // 1/|dt| (compensated rk45: |f| = |kk1|/|dt|)
-  double b[5];       // Cash-Karp b_{s,1..s-1} of this stage
-  double c[7], dc[7];
   // norm weights (variables.f90:49-51)
   double w1, w2, w3;
   // fields (device pointers; complex interleaved)

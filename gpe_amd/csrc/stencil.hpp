@@ -433,7 +433,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
     } else {
       cx<T> u = r.ps;
 #pragma unroll
-      for (int m = 0; m < NIN; ++m) u = axpy<COMP>((T)A.b[m], r.kk[m], u);
+      for (int m = 0; m < NIN; ++m) u = axpy<COMP>((T)ck_b(KIND - K_CK_1 + 1, m), r.kk[m], u);
       return u;
     }
   };
@@ -446,13 +446,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
       F[0] = r.ps;
       F[1] = r.ac + rdiv(r.kk[0], A.d3, 1.0 / 3.0);  // (kk1/6 + kk2/3) + kk3/3
     } else if constexpr (KIND == K_CK_6) {
-      cx<T> pe = rmul((T)A.dc[1], r.kk[0]);
+      cx<T> pe = rmul((T)ck_dc(1), r.kk[0]);
 #pragma unroll
-      for (int m = 1; m < 5; ++m) pe = axpy<COMP>((T)A.dc[m + 1], r.kk[m], pe);
+      for (int m = 1; m < 5; ++m) pe = axpy<COMP>((T)ck_dc(m + 1), r.kk[m], pe);
       if constexpr (COMP) {
-        cx<T> pd = rmul((T)A.c[1], r.kk[0]);  // c2 = c5 = 0
-        pd = axpy<COMP>((T)A.c[3], r.kk[2], pd);
-        pd = axpy<COMP>((T)A.c[4], r.kk[3], pd);
+        cx<T> pd = rmul((T)ck_c(1), r.kk[0]);  // c2 = c5 = 0
+        pd = axpy<COMP>((T)ck_c(3), r.kk[2], pd);
+        pd = axpy<COMP>((T)ck_c(4), r.kk[3], pd);
         F[0] = r.ps;
         F[1] = pd;
         F[2] = pe;
@@ -460,7 +460,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
       } else {
         cx<T> pn = r.ps;
 #pragma unroll
-        for (int m = 0; m < 5; ++m) pn = pn + rmul((T)A.c[m + 1], r.kk[m]);
+        for (int m = 0; m < 5; ++m) pn = pn + rmul((T)ck_c(m + 1), r.kk[m]);
         F[0] = pn;
         F[1] = pe;
       }
@@ -659,16 +659,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
             } else {
               // K_CK_6: new state and the error norm (solve.f90:365-375, 184)
               const cx<T> sc = COMP ? F[0][p][NF - 1] : oc[p][0];
-              const cx<T> err = F[0][p][COMP ? 2 : 1] + rmul((T)A.dc[6], kk);
+              const cx<T> err = F[0][p][COMP ? 2 : 1] + rmul((T)ck_dc(6), kk);
               cx<double> np;
               if constexpr (IS_DBL) {
-                np = F[0][p][0] + rmul((T)A.c[6], kk);
+                np = F[0][p][0] + rmul((T)ck_c(6), kk);
                 stc<T>(A.psi_out, s, np);
                 const double e = cabs_(cdiv(err, sc));
                 acc_emax = (e > acc_emax || e != e) ? e : acc_emax;
               } else {
                 if constexpr (COMP) {
-                  const cx<double> dl = todbl(F[0][p][1]) + rmul(A.c[6], todbl(kk));
+                  const cx<double> dl = todbl(F[0][p][1]) + rmul(ck_c(6), todbl(kk));
                   np = (todbl(F[0][p][0]) + todbl(oc[p][0])) + dl;
                   const cx<T> hi = mk((T)np.re, (T)np.im);
                   stc<T>(A.psi_out, s, hi);
@@ -677,7 +677,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
                   const double e2 = err_ratio2(err, sc);
                   acc_emax = (e2 > acc_emax || e2 != e2) ? e2 : acc_emax;
                 } else {
-                  const cx<T> nf = F[0][p][0] + rmul((T)A.c[6], kk);
+                  const cx<T> nf = F[0][p][0] + rmul((T)ck_c(6), kk);
                   stc<T>(A.psi_out, s, nf);
                   np = todbl(nf);
                   // abs(err/scale) (solve.f90:184) in single precision
