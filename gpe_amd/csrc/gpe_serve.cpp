@@ -19,6 +19,7 @@
 #include <deque>
 #include <map>
 #include <new>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -172,20 +173,36 @@ int gpe_serve_step(gpe_serve *s, int32_t iterations, int32_t *running, int32_t *
   std::vector<int32_t> ids(s->running.size() + 1), status(s->running.size() + 1);
   int32_t n = 0;
   if ((rc = gpe_batch_poll(s->batch, ids.data(), status.data(), (int32_t)ids.size(), &n))) return rc;
+  // Every finished request leaves `running` (the batch reports it once); one
+  // whose result cannot be read back finishes with status -3 and no field,
+  // and the first such error is returned after the others are retired.
+  int first_rc = GPE_OK;
+  std::string first_msg;
   for (int32_t i = 0; i < n; ++i) {
     auto it = s->running.find(ids[i]);
     if (it == s->running.end()) continue;
     Result res;
     res.status = status[i];
     res.P = it->second.P;
+    res.st = it->second.st;
+    rc = GPE_OK;
     try {
       res.field.resize(sites_of(res.P) * 16);
     } catch (const std::bad_alloc &) {
-      return fail(GPE_ENOMEM, "host copy of a result");
+      rc = fail(GPE_ENOMEM, "host copy of a result");
     }
-    if ((rc = gpe_batch_query(s->batch, ids[i], &res.st, nullptr, nullptr, nullptr, nullptr))) return rc;
-    if ((rc = gpe_batch_download(s->batch, ids[i], res.field.data(), GPE_HOST_C128))) return rc;
-    if ((rc = gpe_batch_remove(s->batch, ids[i]))) return rc;
+    if (!rc) rc = gpe_batch_query(s->batch, ids[i], &res.st, nullptr, nullptr, nullptr, nullptr);
+    if (!rc) rc = gpe_batch_download(s->batch, ids[i], res.field.data(), GPE_HOST_C128);
+    const int rc_remove = gpe_batch_remove(s->batch, ids[i]);
+    if (!rc) rc = rc_remove;
+    if (rc) {
+      res.status = -3;
+      res.field.clear();
+      if (!first_rc) {
+        first_rc = rc;
+        first_msg = "result of request " + std::to_string(it->second.ticket) + ": " + gpe::g_err;
+      }
+    }
     s->resident_sites -= it->second.sites;
     s->unreported.emplace_back(it->second.ticket, res.status);
     s->finished.emplace(it->second.ticket, std::move(res));
@@ -193,6 +210,7 @@ int gpe_serve_step(gpe_serve *s, int32_t iterations, int32_t *running, int32_t *
   }
   if (running) *running = (int32_t)s->running.size();
   if (queued) *queued = (int32_t)s->queue.size();
+  if (first_rc) return fail(first_rc, "%s", first_msg.c_str());
   return GPE_OK;
 }
 
@@ -219,7 +237,7 @@ int gpe_serve_result(gpe_serve *s, int64_t ticket, void *psi, int32_t host_type,
   if (r.field.empty()) {
     const int status = r.status;
     s->finished.erase(it);
-    return fail(GPE_ESTATE, "request %lld was not admitted (status %d): no field", (long long)ticket, status);
+    return fail(GPE_ESTATE, "request %lld has no field (status %d)", (long long)ticket, status);
   }
   const size_t n = sites_of(r.P);
   const double *src = reinterpret_cast<const double *>(r.field.data());

@@ -411,11 +411,11 @@ int gpe_serve_submit(gpe_serve *s, const gpe_params *params, const void *psi, in
 int gpe_serve_step(gpe_serve *s, int32_t iterations, int32_t *running, int32_t *queued);
 /* Requests finished since the last poll: tickets and status (1 done, -1 NaN
  * in the rk45 error norm, -2 more than 1000 rejections, -3 could not be
- * admitted: the gpe_serve_step that met it returned the error); n <= cap. */
+ * admitted, or its result could not be read back: the gpe_serve_step that
+ * met it returned the error); n <= cap. */
 int gpe_serve_poll(gpe_serve *s, int64_t *tickets, int32_t *status, int32_t cap, int32_t *n);
 /* Field (host_type) and state of a finished request; releases it
- * (GPE_ESTATE if it has not finished, was collected or was never
- * admitted). */
+ * (GPE_ESTATE if it has not finished, was collected or has status -3). */
 int gpe_serve_result(gpe_serve *s, int64_t ticket, void *psi, int32_t host_type, gpe_step_state *st);
 
 #ifdef __cplusplus
