@@ -171,12 +171,18 @@ class Server:
                         f.result()  # its FFT plan is ready
                     iid = r.add_to(self.batch, device_ic=True)
                     if iid is None:  # does not fit now: wait for memory
+                        if not running:
+                            raise RuntimeError("request %d does not fit in device memory with nothing resident"
+                                               % r.rid)
                         break
                 else:
                     f = futures.pop(r.rid, None)
                     psi = f.result() if f is not None else r.initial_state()
                     iid = self.batch.add(r.cfg, psi, r.steps)
                     if iid is None:  # does not fit now: wait for memory
+                        if not running:
+                            raise RuntimeError("request %d does not fit in device memory with nothing resident"
+                                               % r.rid)
                         futures[r.rid] = _Ready(psi)
                         break
                 del queue[i]
