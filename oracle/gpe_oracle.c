@@ -46,6 +46,18 @@
  *   - the rk45 attempt loop fails on a NaN anywhere in the error field, and
  *     after 1000 rejected attempts, instead of spinning (the reference has no
  *     guard, solve.f90:180-212).
+ *
+ * How each part is pinned (tests/test_oracle*.py, tests/test_refmath.py,
+ * tests/test_linelength_oracle.py, tests/test_zeros_oracle.py,
+ * tests/test_imprint_oracle.py): the stencils and index maps bit for bit
+ * against the reference's derivs.f90 compiled unchanged (golden vectors in
+ * tests/golden/ref_derivs.npz); the right-hand sides, integrators, step
+ * control, norm, energy and mass by plane-wave and cosine-mode closed forms,
+ * the uniform stationary state and scalar restatements of the integrators;
+ * linelength, the zeros passes and imprint_vortex_line by closed forms and
+ * line-by-line transcriptions.  Parity unpinned, at the last-bit level only:
+ * the order of energy()'s flat sum (left to the processor by the Fortran
+ * standard) and glibc tanh against another libm's (eqns 1 and 3 only).
  */
 #include <math.h>
 #include <quadmath.h>
