@@ -522,9 +522,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tx, con
   // Loads in flight: interior and ghosts of plane k+R (two planes ahead
   // for the stages that read few fields per site, whose single plane of
   // prefetch would leave HBM idle between planes), output-time fields of
-  // plane k.
+  // plane k.  The single-precision fourth Cash-Karp stage with one row per
+  // wave also takes two planes: it fits them in 147 VGPRs and its 512^3
+  // sweep ran 1.27 -> 1.03 ms (with two rows per wave two planes were
+  // slower, 1.08 -> 1.29 ms).
   constexpr int NREAD = NIN + (KI::has_acc ? 1 : 0) + 1;
-  constexpr int D = NREAD <= GPE_PREFETCH2_MAX ? 2 : 1;
+  constexpr bool kD2Ck4 = KIND == K_CK_4 && P == 1 && !IS_DBL;
+  constexpr int D = (NREAD <= GPE_PREFETCH2_MAX || kD2Ck4) ? 2 : 1;
   Raw rn[D][P];
   Raw hn[D][HPT];
   cx<T> on[P][NOA];
